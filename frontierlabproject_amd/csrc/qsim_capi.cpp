@@ -1,0 +1,852 @@
+// C ABI of the statevector backend: state objects, validation, dispatch of
+// the tile modes, hipGraph cache, descriptor uploads, reductions, adjoint.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <list>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/qgpusim.h"
+#include "qsim_internal.h"
+#include "qsim_plan.h"
+
+using namespace qsim;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(const std::string &msg)
+{
+    g_err = msg;
+    return -1;
+}
+
+#define HIP_TRY(expr)                                                              \
+    do {                                                                           \
+        hipError_t e_ = (expr);                                                    \
+        if (e_ != hipSuccess) return fail(std::string("HIP error: ") + hipGetErrorString(e_)); \
+    } while (0)
+
+// Pinned staging + device ring for descriptor uploads on one stream.  Host
+// regions are never reused before the stream has drained past their copy: a
+// wrap synchronises the stream first.
+struct Uploader {
+    char *host = nullptr;
+    char *dev = nullptr;
+    size_t cap = 0, head = 0;
+
+    void release()
+    {
+        if (host) hipHostFree(host);
+        if (dev) hipFree(dev);
+        host = dev = nullptr;
+        cap = head = 0;
+    }
+
+    // Returns the device address holding a copy of src[0..bytes).
+    int upload(const void *src, size_t bytes, hipStream_t s, void **out)
+    {
+        bytes = (bytes + 255) & ~size_t(255);
+        if (bytes > cap) {
+            HIP_TRY(hipStreamSynchronize(s));
+            release();
+            cap = std::max(bytes * 2, size_t(4) << 20);
+            HIP_TRY(hipHostMalloc(reinterpret_cast<void **>(&host), cap, hipHostMallocDefault));
+            HIP_TRY(hipMalloc(reinterpret_cast<void **>(&dev), cap));
+            head = 0;
+        }
+        if (head + bytes > cap) {
+            HIP_TRY(hipStreamSynchronize(s));
+            head = 0;
+        }
+        std::memcpy(host + head, src, bytes);
+        HIP_TRY(hipMemcpyAsync(dev + head, host + head, bytes, hipMemcpyHostToDevice, s));
+        *out = dev + head;
+        head += bytes;
+        return 0;
+    }
+};
+
+// Flat descriptor image: passes | steps | probe ids | matrices (state precision).
+struct DescImage {
+    std::vector<char> bytes;
+    size_t off_pass = 0, off_step = 0, off_ids = 0, off_mats = 0;
+
+    template <typename R>
+    void build(const Plan &p)
+    {
+        auto al = [](size_t x) { return (x + 255) & ~size_t(255); };
+        const size_t nm = p.mats.size() / 2;
+        off_pass = 0;
+        off_step = al(off_pass + p.passes.size() * sizeof(PassDev));
+        off_ids = al(off_step + p.steps.size() * sizeof(StepDev));
+        off_mats = al(off_ids + p.probe_ids.size() * sizeof(int32_t));
+        bytes.assign(al(off_mats + nm * sizeof(cx<R>)) + 256, 0);
+        if (!p.passes.empty()) std::memcpy(&bytes[off_pass], p.passes.data(), p.passes.size() * sizeof(PassDev));
+        if (!p.steps.empty()) std::memcpy(&bytes[off_step], p.steps.data(), p.steps.size() * sizeof(StepDev));
+        if (!p.probe_ids.empty())
+            std::memcpy(&bytes[off_ids], p.probe_ids.data(), p.probe_ids.size() * sizeof(int32_t));
+        cx<R> *m = reinterpret_cast<cx<R> *>(&bytes[off_mats]);
+        for (size_t i = 0; i < nm; ++i) m[i] = cx<R>{(R)p.mats[2 * i], (R)p.mats[2 * i + 1]};
+    }
+};
+
+struct GraphEntry {
+    std::vector<char> key;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    void *desc = nullptr;
+};
+
+size_t amp_bytes(int dtype) { return dtype == QSIM_C64 ? 8 : 16; }
+
+}  // namespace
+
+struct qsim_state {
+    int n = 0;
+    int dtype = QSIM_C64;
+    uint64_t dim = 0;
+    int device = 0;
+    void *psi = nullptr;
+    void *tmp = nullptr;
+    hipStream_t stream = nullptr;
+    int K = 12, cmin = 4;
+    Uploader up;
+    std::list<GraphEntry> graphs;  // most recent first
+    double *d_small = nullptr;     // reduction scratch (partials)
+    size_t small_cap = 0;          // doubles
+    double *d_probe = nullptr;     // probe partials
+    size_t probe_cap = 0;          // doubles
+    int64_t counters[5] = {0, 0, 0, 0, 0};
+};
+
+namespace {
+
+constexpr size_t kMaxGraphs = 64;
+
+int ensure_tmp(qsim_state *st)
+{
+    if (!st->tmp) HIP_TRY(hipMalloc(&st->tmp, st->dim * amp_bytes(st->dtype)));
+    return 0;
+}
+
+int ensure_small(qsim_state *st, size_t doubles)
+{
+    if (doubles <= st->small_cap) return 0;
+    HIP_TRY(hipStreamSynchronize(st->stream));
+    if (st->d_small) hipFree(st->d_small);
+    st->d_small = nullptr;
+    HIP_TRY(hipMalloc(reinterpret_cast<void **>(&st->d_small), doubles * sizeof(double)));
+    st->small_cap = doubles;
+    return 0;
+}
+
+int ensure_probe(qsim_state *st, size_t doubles)
+{
+    if (doubles <= st->probe_cap) return 0;
+    HIP_TRY(hipStreamSynchronize(st->stream));
+    if (st->d_probe) hipFree(st->d_probe);
+    st->d_probe = nullptr;
+    HIP_TRY(hipMalloc(reinterpret_cast<void **>(&st->d_probe), doubles * sizeof(double)));
+    st->probe_cap = doubles;
+    return 0;
+}
+
+void drop_graphs(qsim_state *st)
+{
+    if (!st->graphs.empty()) hipStreamSynchronize(st->stream);
+    for (auto &g : st->graphs) {
+        if (g.exec) hipGraphExecDestroy(g.exec);
+        if (g.graph) hipGraphDestroy(g.graph);
+        if (g.desc) hipFree(g.desc);
+    }
+    st->graphs.clear();
+}
+
+int check_ops(int n, const qsim_op *ops, int nops, uint64_t mat_count, bool allow_probe)
+{
+    if (nops < 0) return fail("apply_tile_dev: negative op count");
+    if (nops > 0 && !ops) return fail("apply_tile_dev: null op array");
+    for (int i = 0; i < nops; ++i) {
+        const qsim_op &o = ops[i];
+        if (o.kind == QSIM_OP_1Q || (allow_probe && o.kind == QSIM_OP_PROBE)) {
+            if (o.q0 < 0 || o.q0 >= n) return fail("apply_tile_dev: invalid q0");
+            if (o.mat_offset < 0 || (uint64_t)o.mat_offset + 4 > mat_count)
+                return fail("apply_tile_dev: matrix offset out of range");
+        } else if (o.kind == QSIM_OP_2Q) {
+            if (o.q0 < 0 || o.q0 >= n || o.q1 < 0 || o.q1 >= n) return fail("apply_tile_dev: invalid q0/q1");
+            if (o.q0 == o.q1) return fail("apply_tile_dev: q0 == q1");
+            if (o.mat_offset < 0 || (uint64_t)o.mat_offset + 16 > mat_count)
+                return fail("apply_tile_dev: matrix offset out of range");
+        } else {
+            return fail("apply_tile_dev: unknown op kind: " + std::to_string(o.kind));
+        }
+    }
+    return 0;
+}
+
+std::vector<PlanOp> to_plan_ops(const qsim_op *ops, int nops)
+{
+    std::vector<PlanOp> v(nops);
+    for (int i = 0; i < nops; ++i) {
+        v[i].kind = ops[i].kind;
+        v[i].q0 = ops[i].q0;
+        v[i].q1 = ops[i].kind == QSIM_OP_2Q ? ops[i].q1 : -1;
+        v[i].mat_off = ops[i].mat_offset;
+        v[i].probe = ops[i].kind == QSIM_OP_PROBE ? ops[i].q1 : -1;
+    }
+    return v;
+}
+
+template <typename R>
+int run_sequential(qsim_state *st, const qsim_op *ops, int nops, const void *mats)
+{
+    const cx<R> *m = static_cast<const cx<R> *>(mats);
+    cx<R> *psi = static_cast<cx<R> *>(st->psi);
+    for (int i = 0; i < nops; ++i) {
+        const qsim_op &o = ops[i];
+        if (o.kind == QSIM_OP_1Q) {
+            Mat1<R> U;
+            std::memcpy(U.u, m + o.mat_offset, sizeof(U));
+            HIP_TRY(launch_1q<R>(psi, st->n, o.q0, U, st->stream));
+        } else {
+            Mat2<R> U;
+            std::memcpy(U.u, m + o.mat_offset, sizeof(U));
+            HIP_TRY(launch_2q<R>(psi, st->n, o.q0, o.q1, U, st->stream));
+        }
+        st->counters[0]++;
+    }
+    return 0;
+}
+
+// Enqueue the passes of `plan` whose descriptor image lives at `dev`.
+template <typename R>
+int enqueue_passes(qsim_state *st, const Plan &plan, const DescImage &img, char *dev, void *lam,
+                   double *probe_out)
+{
+    const PassDev *dp = reinterpret_cast<const PassDev *>(dev + img.off_pass);
+    const StepDev *ds = reinterpret_cast<const StepDev *>(dev + img.off_step);
+    const int32_t *di = reinterpret_cast<const int32_t *>(dev + img.off_ids);
+    const cx<R> *dm = reinterpret_cast<const cx<R> *>(dev + img.off_mats);
+    for (size_t p = 0; p < plan.passes.size(); ++p) {
+        const PassDev &hp = plan.passes[p];
+        const int nblocks = 1 << (st->n - hp.K);
+        HIP_TRY(launch_tile_pass<R>(static_cast<cx<R> *>(st->psi), static_cast<cx<R> *>(lam), st->n, hp, dp + p, ds,
+                                    dm, st->d_probe, st->stream));
+        st->counters[0]++;
+        st->counters[3]++;
+        if (lam && hp.nprobe > 0) {
+            HIP_TRY(launch_probe_reduce(st->d_probe, nblocks, hp.nprobe, di + hp.probe_off, probe_out, st->stream));
+            st->counters[0]++;
+        }
+    }
+    return 0;
+}
+
+int plan_for(qsim_state *st, const qsim_op *ops, int nops, const void *mats, int max_probes, Plan &plan)
+{
+    std::vector<PlanOp> pops = to_plan_ops(ops, nops);
+    std::string err;
+    if (!plan_passes(st->n, pops.data(), nops, mats, (int)amp_bytes(st->dtype), st->K, st->cmin, max_probes, plan,
+                     err))
+        return fail(err);
+    return 0;
+}
+
+template <typename R>
+int run_chunked(qsim_state *st, const qsim_op *ops, int nops, const void *mats, uint64_t mat_count)
+{
+    (void)mat_count;
+    Plan plan;
+    if (plan_for(st, ops, nops, mats, 1, plan)) return -1;
+    DescImage img;
+    img.build<R>(plan);
+    void *dev = nullptr;
+    if (st->up.upload(img.bytes.data(), img.bytes.size(), st->stream, &dev)) return -1;
+    return enqueue_passes<R>(st, plan, img, static_cast<char *>(dev), nullptr, nullptr);
+}
+
+template <typename R>
+int run_graphed(qsim_state *st, const qsim_op *ops, int nops, const void *mats, uint64_t mat_count)
+{
+    // key: buffer address, geometry, ops, matrices
+    std::vector<char> key;
+    auto put = [&key](const void *p, size_t b) {
+        const char *c = static_cast<const char *>(p);
+        key.insert(key.end(), c, c + b);
+    };
+    put(&st->psi, sizeof(void *));
+    put(&st->K, sizeof(int));
+    put(&st->cmin, sizeof(int));
+    put(ops, sizeof(qsim_op) * nops);
+    put(mats, sizeof(cx<R>) * mat_count);
+    for (auto it = st->graphs.begin(); it != st->graphs.end(); ++it) {
+        if (it->key == key) {
+            st->graphs.splice(st->graphs.begin(), st->graphs, it);
+            HIP_TRY(hipGraphLaunch(st->graphs.front().exec, st->stream));
+            st->counters[1]++;
+            st->counters[3] += 0;
+            return 0;
+        }
+    }
+    Plan plan;
+    if (plan_for(st, ops, nops, mats, 1, plan)) return -1;
+    DescImage img;
+    img.build<R>(plan);
+    GraphEntry e;
+    e.key.swap(key);
+    HIP_TRY(hipMalloc(&e.desc, img.bytes.size()));
+    HIP_TRY(hipMemcpyAsync(e.desc, img.bytes.data(), img.bytes.size(), hipMemcpyHostToDevice, st->stream));
+    HIP_TRY(hipStreamSynchronize(st->stream));
+    HIP_TRY(hipStreamBeginCapture(st->stream, hipStreamCaptureModeThreadLocal));
+    int rc = enqueue_passes<R>(st, plan, img, static_cast<char *>(e.desc), nullptr, nullptr);
+    hipGraph_t g = nullptr;
+    hipError_t ec = hipStreamEndCapture(st->stream, &g);
+    if (rc || ec != hipSuccess) {
+        if (g) hipGraphDestroy(g);
+        hipFree(e.desc);
+        if (rc) return rc;
+        return fail(std::string("HIP error: ") + hipGetErrorString(ec));
+    }
+    e.graph = g;
+    hipError_t ei = hipGraphInstantiate(&e.exec, g, nullptr, nullptr, 0);
+    if (ei != hipSuccess) {
+        hipGraphDestroy(g);
+        hipFree(e.desc);
+        return fail(std::string("HIP error: ") + hipGetErrorString(ei));
+    }
+    st->counters[2]++;
+    st->graphs.push_front(std::move(e));
+    HIP_TRY(hipGraphLaunch(st->graphs.front().exec, st->stream));
+    st->counters[1]++;
+    while (st->graphs.size() > kMaxGraphs) {
+        HIP_TRY(hipStreamSynchronize(st->stream));
+        GraphEntry &old = st->graphs.back();
+        hipGraphExecDestroy(old.exec);
+        hipGraphDestroy(old.graph);
+        hipFree(old.desc);
+        st->graphs.pop_back();
+    }
+    return 0;
+}
+
+// Chunk planning needs at least kG qubits; tiny states use the gate streams.
+bool chunk_ok(const qsim_state *st) { return st->n >= 6; }
+
+template <typename R>
+int apply_ops_t(qsim_state *st, int mode, const qsim_op *ops, int nops, const void *mats, uint64_t mat_count)
+{
+    if (nops == 0) return 0;
+    if (mode == QSIM_SEQUENTIAL || !chunk_ok(st)) return run_sequential<R>(st, ops, nops, mats);
+    if (mode == QSIM_COOPERATIVE) return run_chunked<R>(st, ops, nops, mats, mat_count);
+    if (mode == QSIM_GRAPHED) return run_graphed<R>(st, ops, nops, mats, mat_count);
+    return fail("apply_tile: unknown tile mode " + std::to_string(mode));
+}
+
+int build_perm_args(int n, const int32_t *map, int smax, PermArgs &P)
+{
+    int inv[64];
+    for (int p = 0; p < n; ++p) inv[map[p]] = p;
+    int a = std::min(n, smax);
+    uint64_t S = 0;
+    for (; a >= 1; --a) {
+        S = (a >= 64 ? ~0ull : ((1ull << a) - 1));
+        for (int k = 0; k < a; ++k) S |= 1ull << inv[k];
+        if (__builtin_popcountll(S) <= smax) break;
+    }
+    std::memset(&P, 0, sizeof(P));
+    P.n = n;
+    P.a = P.b = a;
+    P.s = __builtin_popcountll(S);
+    int sin[64], ns = 0, comp = 0;
+    for (int p = 0; p < n; ++p) {
+        if (S >> p & 1) sin[ns++] = p;
+        else {
+            P.comp_in[comp] = p;
+            P.comp_out[comp] = map[p];
+            ++comp;
+        }
+    }
+    P.ncomp = comp;
+    int sout[64];
+    for (int k = 0; k < ns; ++k) sout[k] = map[sin[k]];
+    std::sort(sout, sout + ns);
+    for (int k = 0; k < ns - P.a; ++k) P.sin_hi[k] = sin[P.a + k];
+    for (int k = 0; k < ns - P.b; ++k) P.sout_hi[k] = sout[P.b + k];
+    for (int k = 0; k < ns; ++k) {
+        const int in_pos = inv[sout[k]];
+        int r = 0;
+        while (sin[r] != in_pos) ++r;
+        P.w2r[k] = r;
+    }
+    return 0;
+}
+
+int permute_map_impl(int n, const int32_t *lo, const int32_t *ln, int32_t *map)
+{
+    std::vector<int> new_pos(n, -1);
+    for (int p = 0; p < n; ++p) {
+        const int gq = ln[p];
+        if (gq < 0 || gq >= n) return fail("layout_new contains invalid qubit id");
+        if (new_pos[gq] != -1) return fail("layout_new is not a permutation");
+        new_pos[gq] = p;
+    }
+    for (int p = 0; p < n; ++p) {
+        const int gq = lo[p];
+        if (gq < 0 || gq >= n) return fail("layout_old contains invalid qubit id");
+        if (new_pos[gq] < 0) return fail("layout_old contains qubit not in layout_new");
+        map[p] = new_pos[gq];
+    }
+    return 0;
+}
+
+template <typename R>
+int dot_t(qsim_state *st, const void *a, const void *b, double *re, double *im)
+{
+    if (ensure_small(st, 2 * (size_t)max_dot_blocks() + 8)) return -1;
+    int nb = 0;
+    HIP_TRY(launch_dot<R>(static_cast<const cx<R> *>(a), static_cast<const cx<R> *>(b), st->dim, st->d_small, &nb,
+                          st->stream));
+    double *out = st->d_small + 2 * (size_t)max_dot_blocks();
+    HIP_TRY(launch_sum_partials(st->d_small, nb, 2, out, st->stream));
+    double h[2];
+    HIP_TRY(hipMemcpyAsync(h, out, sizeof(h), hipMemcpyDeviceToHost, st->stream));
+    HIP_TRY(hipStreamSynchronize(st->stream));
+    st->counters[0] += 2;
+    *re = h[0];
+    if (im) *im = h[1];
+    return 0;
+}
+
+// Pauli chunk passes.  lam == nullptr: expectation only.
+template <typename R>
+int pauli_t(qsim_state *st, const qsim_pauli *terms, int nterms, void *lam, double *re, double *im)
+{
+    std::vector<uint64_t> xm(nterms);
+    for (int t = 0; t < nterms; ++t) xm[t] = terms[t].xmask;
+    std::vector<PauliPass> passes;
+    std::string err;
+    if (!plan_pauli(st->n, xm.data(), nterms, st->K, st->cmin, passes, err)) return fail(err);
+    // descriptor image: per pass one PassDev followed by its terms
+    std::vector<char> img;
+    std::vector<size_t> pass_off, term_off;
+    for (auto &pp : passes) {
+        pass_off.push_back(img.size());
+        img.resize(img.size() + ((sizeof(PassDev) + 255) & ~size_t(255)));
+        std::memcpy(&img[pass_off.back()], &pp.pd, sizeof(PassDev));
+        term_off.push_back(img.size());
+        const size_t tb = pp.terms.size() * sizeof(PauliDev);
+        img.resize(img.size() + ((tb + 255) & ~size_t(255)) + 256);
+        for (size_t k = 0; k < pp.terms.size(); ++k) {
+            const qsim_pauli &T = terms[pp.terms[k]];
+            PauliDev d{};
+            d.xmask = T.xmask;
+            d.zmask = T.zmask;
+            d.xloc = pp.xloc[k];
+            d.cre = T.coef_re;
+            d.cim = T.coef_im;
+            std::memcpy(&img[term_off.back() + k * sizeof(PauliDev)], &d, sizeof(d));
+        }
+    }
+    void *dev = nullptr;
+    if (!img.empty() && st->up.upload(img.data(), img.size(), st->stream, &dev)) return -1;
+    size_t maxblocks = 1;
+    for (auto &pp : passes) maxblocks = std::max(maxblocks, size_t(1) << (st->n - pp.pd.K));
+    if (ensure_small(st, 2 * maxblocks + 2 * passes.size() + 8)) return -1;
+    double *sums = st->d_small + 2 * maxblocks;
+    for (size_t p = 0; p < passes.size(); ++p) {
+        const PauliPass &pp = passes[p];
+        int mode = PAULI_EXPECT;
+        if (lam) mode |= (p == 0 ? PAULI_WRITE : PAULI_ACC);
+        const int nblocks = 1 << (st->n - pp.pd.K);
+        char *base = static_cast<char *>(dev);
+        HIP_TRY(launch_pauli_chunk<R>(static_cast<const cx<R> *>(st->psi), static_cast<cx<R> *>(lam), st->n, pp.pd,
+                                      reinterpret_cast<const PassDev *>(base + pass_off[p]),
+                                      reinterpret_cast<const PauliDev *>(base + term_off[p]), (int)pp.terms.size(),
+                                      mode, st->d_small, st->stream));
+        HIP_TRY(launch_sum_partials(st->d_small, nblocks, 2, sums + 2 * p, st->stream));
+        st->counters[0] += 2;
+        st->counters[3]++;
+    }
+    std::vector<double> h(2 * passes.size() + 2, 0.0);
+    if (!passes.empty())
+        HIP_TRY(hipMemcpyAsync(h.data(), sums, 2 * passes.size() * sizeof(double), hipMemcpyDeviceToHost, st->stream));
+    HIP_TRY(hipStreamSynchronize(st->stream));
+    double r = 0, i = 0;
+    for (size_t p = 0; p < passes.size(); ++p) {
+        r += h[2 * p];
+        i += h[2 * p + 1];
+    }
+    if (re) *re = r;
+    if (im) *im = i;
+    return 0;
+}
+
+template <typename R>
+int adjoint_t(qsim_state *st, const qsim_pauli *terms, int nterms, const qsim_op *ops, int nops, const void *mats,
+              uint64_t mat_count, int nprobes, double *probes_out, double *expect_out)
+{
+    if (ensure_tmp(st)) return -1;
+    double er = 0, ei = 0;
+    if (pauli_t<R>(st, terms, nterms, st->tmp, &er, &ei)) return -1;
+    if (expect_out) *expect_out = er;
+    (void)mat_count;
+    Plan plan;
+    if (plan_for(st, ops, nops, mats, kMaxProbes, plan)) return -1;
+    size_t need = 1;
+    for (auto &p : plan.passes) need = std::max(need, (size_t(1) << (st->n - p.K)) * (size_t)std::max(1, p.nprobe));
+    if (ensure_probe(st, need)) return -1;
+    double *d_out = nullptr;
+    HIP_TRY(hipMalloc(reinterpret_cast<void **>(&d_out), sizeof(double) * std::max(1, nprobes)));
+    std::unique_ptr<void, void (*)(void *)> guard(d_out, [](void *p) { hipFree(p); });
+    HIP_TRY(hipMemsetAsync(d_out, 0, sizeof(double) * std::max(1, nprobes), st->stream));
+    DescImage img;
+    img.build<R>(plan);
+    void *dev = nullptr;
+    if (st->up.upload(img.bytes.data(), img.bytes.size(), st->stream, &dev)) return -1;
+    if (enqueue_passes<R>(st, plan, img, static_cast<char *>(dev), st->tmp, d_out)) return -1;
+    if (nprobes > 0)
+        HIP_TRY(hipMemcpyAsync(probes_out, d_out, sizeof(double) * nprobes, hipMemcpyDeviceToHost, st->stream));
+    HIP_TRY(hipStreamSynchronize(st->stream));
+    return 0;
+}
+
+}  // namespace
+
+// ============================================================================
+// exported C ABI
+// ============================================================================
+extern "C" {
+
+const char *qsim_last_error(void) { return g_err.c_str(); }
+
+int qsim_abi_version(void) { return 1; }
+
+int qsim_create(int num_qubits, int dtype, qsim_state **out)
+{
+    if (!out) return fail("qsim_create: null output");
+    *out = nullptr;
+    if (num_qubits < 1 || num_qubits > 40) return fail("num_qubits must be in [1, 40]");
+    if (dtype != QSIM_C64 && dtype != QSIM_C128) return fail("unknown dtype");
+    std::unique_ptr<qsim_state> st(new qsim_state());
+    st->n = num_qubits;
+    st->dtype = dtype;
+    st->dim = 1ull << num_qubits;
+    st->K = dtype == QSIM_C64 ? 12 : 11;
+    st->cmin = dtype == QSIM_C64 ? 4 : 3;
+    HIP_TRY(hipGetDevice(&st->device));
+    static bool prepared = false;
+    if (!prepared) {
+        HIP_TRY(prepare_kernels());
+        prepared = true;
+    }
+    HIP_TRY(hipStreamCreateWithFlags(&st->stream, hipStreamNonBlocking));
+    hipError_t e = hipMalloc(&st->psi, st->dim * amp_bytes(dtype));
+    if (e != hipSuccess) {
+        hipStreamDestroy(st->stream);
+        return fail(std::string("HIP error: ") + hipGetErrorString(e) + " (out of memory allocating statevector)");
+    }
+    if (qsim_reset(st.get())) {
+        hipFree(st->psi);
+        hipStreamDestroy(st->stream);
+        return -1;
+    }
+    *out = st.release();
+    return 0;
+}
+
+int qsim_destroy(qsim_state *st)
+{
+    if (!st) return 0;
+    hipStreamSynchronize(st->stream);
+    drop_graphs(st);
+    st->up.release();
+    if (st->psi) hipFree(st->psi);
+    if (st->tmp) hipFree(st->tmp);
+    if (st->d_small) hipFree(st->d_small);
+    if (st->d_probe) hipFree(st->d_probe);
+    if (st->stream) hipStreamDestroy(st->stream);
+    delete st;
+    return 0;
+}
+
+int qsim_num_qubits(const qsim_state *st) { return st ? st->n : -1; }
+uint64_t qsim_dim(const qsim_state *st) { return st ? st->dim : 0; }
+int qsim_dtype(const qsim_state *st) { return st ? st->dtype : -1; }
+
+int qsim_device_ptr(qsim_state *st, void **out)
+{
+    if (!st || !out) return fail("null argument");
+    *out = st->psi;
+    return 0;
+}
+
+int qsim_reset(qsim_state *st)
+{
+    if (!st) return fail("null state");
+    if (st->dtype == QSIM_C64)
+        HIP_TRY(launch_init_basis<float>(static_cast<cx<float> *>(st->psi), st->dim, 0, st->stream));
+    else
+        HIP_TRY(launch_init_basis<double>(static_cast<cx<double> *>(st->psi), st->dim, 0, st->stream));
+    st->counters[0]++;
+    return 0;
+}
+
+int qsim_from_host(qsim_state *st, const void *src, uint64_t count)
+{
+    if (!st) return fail("null state");
+    if (count != st->dim) return fail("Input array must have size 2^num_qubits");
+    HIP_TRY(hipMemcpyAsync(st->psi, src, count * amp_bytes(st->dtype), hipMemcpyHostToDevice, st->stream));
+    HIP_TRY(hipStreamSynchronize(st->stream));
+    return 0;
+}
+
+int qsim_to_host(qsim_state *st, void *dst, uint64_t count)
+{
+    if (!st) return fail("null state");
+    if (count != st->dim) return fail("Output array must have size 2^num_qubits");
+    HIP_TRY(hipMemcpyAsync(dst, st->psi, count * amp_bytes(st->dtype), hipMemcpyDeviceToHost, st->stream));
+    HIP_TRY(hipStreamSynchronize(st->stream));
+    return 0;
+}
+
+int qsim_synchronize(qsim_state *st)
+{
+    if (!st) return fail("null state");
+    HIP_TRY(hipStreamSynchronize(st->stream));
+    return 0;
+}
+
+int qsim_permute_map(int num_qubits, const int32_t *layout_old, const int32_t *layout_new, int32_t *map_out)
+{
+    if (num_qubits < 1 || num_qubits > 63 || !layout_old || !layout_new || !map_out)
+        return fail("layout_old/layout_new must have length num_qubits");
+    return permute_map_impl(num_qubits, layout_old, layout_new, map_out);
+}
+
+int qsim_permute(qsim_state *st, const int32_t *layout_old, const int32_t *layout_new, int num_qubits)
+{
+    if (!st) return fail("null state");
+    if (num_qubits != st->n) return fail("layout_old/layout_new must have length num_qubits");
+    int32_t map[64];
+    if (permute_map_impl(st->n, layout_old, layout_new, map)) return -1;
+    bool ident = true;
+    for (int p = 0; p < st->n; ++p) ident &= map[p] == p;
+    if (ident) return 0;
+    if (ensure_tmp(st)) return -1;
+    PermArgs P;
+    build_perm_args(st->n, map, st->dtype == QSIM_C64 ? 12 : 11, P);
+    if (st->dtype == QSIM_C64)
+        HIP_TRY(launch_permute<float>(static_cast<cx<float> *>(st->psi), static_cast<cx<float> *>(st->tmp), P,
+                                      st->stream));
+    else
+        HIP_TRY(launch_permute<double>(static_cast<cx<double> *>(st->psi), static_cast<cx<double> *>(st->tmp), P,
+                                       st->stream));
+    std::swap(st->psi, st->tmp);
+    st->counters[0]++;
+    st->counters[4]++;
+    return 0;
+}
+
+int qsim_validate_ops(int num_qubits, const qsim_op *ops, int nops, uint64_t mat_count)
+{
+    return check_ops(num_qubits, ops, nops, mat_count, false);
+}
+
+int qsim_apply_ops(qsim_state *st, int mode, const qsim_op *ops, int nops, const void *mats, uint64_t mat_count)
+{
+    if (!st) return fail("null state");
+    if (check_ops(st->n, ops, nops, mat_count, false)) return -1;
+    if (st->dtype == QSIM_C64) return apply_ops_t<float>(st, mode, ops, nops, mats, mat_count);
+    return apply_ops_t<double>(st, mode, ops, nops, mats, mat_count);
+}
+
+int qsim_apply_1q(qsim_state *st, int target_qubit, const void *U)
+{
+    if (!st) return fail("null state");
+    if (target_qubit < 0 || target_qubit >= st->n) return fail("Invalid target qubit");
+    if (st->dtype == QSIM_C64) {
+        Mat1<float> M;
+        std::memcpy(M.u, U, sizeof(M));
+        HIP_TRY(launch_1q<float>(static_cast<cx<float> *>(st->psi), st->n, target_qubit, M, st->stream));
+    } else {
+        Mat1<double> M;
+        std::memcpy(M.u, U, sizeof(M));
+        HIP_TRY(launch_1q<double>(static_cast<cx<double> *>(st->psi), st->n, target_qubit, M, st->stream));
+    }
+    st->counters[0]++;
+    return 0;
+}
+
+int qsim_apply_2q(qsim_state *st, int q0, int q1, const void *U)
+{
+    if (!st) return fail("null state");
+    if (q0 < 0 || q0 >= st->n || q1 < 0 || q1 >= st->n) return fail("Invalid qubit indices");
+    if (q0 == q1) return fail("q0 and q1 must be different");
+    if (st->dtype == QSIM_C64) {
+        Mat2<float> M;
+        std::memcpy(M.u, U, sizeof(M));
+        HIP_TRY(launch_2q<float>(static_cast<cx<float> *>(st->psi), st->n, q0, q1, M, st->stream));
+    } else {
+        Mat2<double> M;
+        std::memcpy(M.u, U, sizeof(M));
+        HIP_TRY(launch_2q<double>(static_cast<cx<double> *>(st->psi), st->n, q0, q1, M, st->stream));
+    }
+    st->counters[0]++;
+    return 0;
+}
+
+int qsim_apply_diagonal_1q(qsim_state *st, int q, double p0r, double p0i, double p1r, double p1i)
+{
+    if (!st) return fail("null state");
+    if (q < 0 || q >= st->n) return fail("Invalid target qubit");
+    if (st->dtype == QSIM_C64)
+        HIP_TRY(launch_diag1q<float>(static_cast<cx<float> *>(st->psi), st->n, q, cx<float>{(float)p0r, (float)p0i},
+                                     cx<float>{(float)p1r, (float)p1i}, st->stream));
+    else
+        HIP_TRY(launch_diag1q<double>(static_cast<cx<double> *>(st->psi), st->n, q, cx<double>{p0r, p0i},
+                                      cx<double>{p1r, p1i}, st->stream));
+    st->counters[0]++;
+    return 0;
+}
+
+int qsim_apply_1q_raw(void *psi, int dtype, int n, int q, const void *U, void *stream)
+{
+    if (!psi) return fail("device array has null pointer");
+    if (n < 1 || n > 40) return fail("num_qubits must be in [1, 40]");
+    if (q < 0 || q >= n) return fail("Invalid target qubit");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (dtype == QSIM_C64) {
+        Mat1<float> M;
+        std::memcpy(M.u, U, sizeof(M));
+        HIP_TRY(launch_1q<float>(static_cast<cx<float> *>(psi), n, q, M, s));
+    } else if (dtype == QSIM_C128) {
+        Mat1<double> M;
+        std::memcpy(M.u, U, sizeof(M));
+        HIP_TRY(launch_1q<double>(static_cast<cx<double> *>(psi), n, q, M, s));
+    } else {
+        return fail("unknown dtype");
+    }
+    HIP_TRY(hipStreamSynchronize(s));
+    return 0;
+}
+
+int qsim_apply_2q_raw(void *psi, int dtype, int n, int q0, int q1, const void *U, void *stream)
+{
+    if (!psi) return fail("device array has null pointer");
+    if (n < 2 || n > 40) return fail("num_qubits must be in [2, 40]");
+    if (q0 < 0 || q0 >= n || q1 < 0 || q1 >= n) return fail("Invalid qubit indices");
+    if (q0 == q1) return fail("q0 and q1 must be different");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (dtype == QSIM_C64) {
+        Mat2<float> M;
+        std::memcpy(M.u, U, sizeof(M));
+        HIP_TRY(launch_2q<float>(static_cast<cx<float> *>(psi), n, q0, q1, M, s));
+    } else if (dtype == QSIM_C128) {
+        Mat2<double> M;
+        std::memcpy(M.u, U, sizeof(M));
+        HIP_TRY(launch_2q<double>(static_cast<cx<double> *>(psi), n, q0, q1, M, s));
+    } else {
+        return fail("unknown dtype");
+    }
+    HIP_TRY(hipStreamSynchronize(s));
+    return 0;
+}
+
+int qsim_norm2(qsim_state *st, double *out)
+{
+    if (!st || !out) return fail("null argument");
+    if (st->dtype == QSIM_C64) return dot_t<float>(st, st->psi, st->psi, out, nullptr);
+    return dot_t<double>(st, st->psi, st->psi, out, nullptr);
+}
+
+int qsim_overlap(qsim_state *a, qsim_state *b, double *re, double *im)
+{
+    if (!a || !b || !re || !im) return fail("null argument");
+    if (a->n != b->n || a->dtype != b->dtype) return fail("overlap: states differ in size or precision");
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    if (a->dtype == QSIM_C64) return dot_t<float>(a, a->psi, b->psi, re, im);
+    return dot_t<double>(a, a->psi, b->psi, re, im);
+}
+
+int qsim_scale(qsim_state *st, double factor)
+{
+    if (!st) return fail("null state");
+    if (st->dtype == QSIM_C64)
+        HIP_TRY(launch_scale<float>(static_cast<cx<float> *>(st->psi), st->dim, factor, st->stream));
+    else
+        HIP_TRY(launch_scale<double>(static_cast<cx<double> *>(st->psi), st->dim, factor, st->stream));
+    st->counters[0]++;
+    return 0;
+}
+
+int qsim_expect_pauli(qsim_state *st, const qsim_pauli *terms, int nterms, double *re, double *im)
+{
+    if (!st) return fail("null state");
+    if (nterms < 0 || (nterms > 0 && !terms)) return fail("expect_pauli: bad term array");
+    if (st->n < 6) return fail("expect_pauli: needs at least 6 qubits");
+    if (st->dtype == QSIM_C64) return pauli_t<float>(st, terms, nterms, nullptr, re, im);
+    return pauli_t<double>(st, terms, nterms, nullptr, re, im);
+}
+
+int qsim_adjoint(qsim_state *st, const qsim_pauli *terms, int nterms, const qsim_op *rev_ops, int nops,
+                 const void *mats, uint64_t mat_count, int nprobes, double *probes_out, double *expect_out)
+{
+    if (!st) return fail("null state");
+    if (st->n < 6) return fail("adjoint: needs at least 6 qubits");
+    if (nprobes < 0 || (nprobes > 0 && !probes_out)) return fail("adjoint: bad probe output");
+    if (check_ops(st->n, rev_ops, nops, mat_count, true)) return -1;
+    for (int i = 0; i < nops; ++i)
+        if (rev_ops[i].kind == QSIM_OP_PROBE && (rev_ops[i].q1 < 0 || rev_ops[i].q1 >= nprobes))
+            return fail("adjoint: probe slot out of range");
+    if (st->dtype == QSIM_C64)
+        return adjoint_t<float>(st, terms, nterms, rev_ops, nops, mats, mat_count, nprobes, probes_out, expect_out);
+    return adjoint_t<double>(st, terms, nterms, rev_ops, nops, mats, mat_count, nprobes, probes_out, expect_out);
+}
+
+int qsim_set_tile_params(qsim_state *st, int K, int cmin)
+{
+    if (!st) return fail("null state");
+    const int kmax = st->dtype == QSIM_C64 ? 13 : 12;
+    if (K < 6 || K > kmax) return fail("K out of range");
+    if (cmin < 0 || cmin > K - 2) return fail("cmin out of range");
+    st->K = K;
+    st->cmin = cmin;
+    return 0;
+}
+
+int qsim_get_counters(const qsim_state *st, int64_t *out, int n)
+{
+    if (!st || !out) return fail("null argument");
+    const int m = std::min(n, 5);
+    for (int i = 0; i < m; ++i) out[i] = st->counters[i];
+    return m;
+}
+
+int qsim_plan_describe(int num_qubits, int dtype, const qsim_op *ops, int nops, const void *mats,
+                       uint64_t mat_count, int K, int cmin, int32_t *desc, int desc_cap, int32_t *order)
+{
+    if (dtype != QSIM_C64 && dtype != QSIM_C128) return fail("unknown dtype");
+    if (check_ops(num_qubits, ops, nops, mat_count, true)) return -1;
+    std::vector<PlanOp> pops = to_plan_ops(ops, nops);
+    Plan plan;
+    std::string err;
+    if (!plan_passes(num_qubits, pops.data(), nops, mats, (int)amp_bytes(dtype), K, cmin, kMaxProbes, plan, err))
+        return fail(err);
+    int w = 0;
+    for (const PassDev &p : plan.passes) {
+        if (w + 3 + p.K > desc_cap) return fail("plan_describe: descriptor buffer too small");
+        desc[w++] = p.K;
+        desc[w++] = p.cfast;
+        desc[w++] = p.nsteps;
+        for (int k = 0; k < p.K; ++k) desc[w++] = p.pos[k];
+    }
+    for (size_t i = 0; i < plan.order.size(); ++i) order[i] = plan.order[i];
+    return (int)plan.passes.size();
+}
+
+}  // extern "C"

@@ -1,0 +1,78 @@
+// Internal declarations shared by the kernels and the C ABI layer.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "qsim_plan.h"
+
+namespace qsim {
+
+template <typename R> struct cx { R re, im; };
+
+template <typename R> struct Mat1 { cx<R> u[4]; };
+template <typename R> struct Mat2 { cx<R> u[16]; };
+
+constexpr int kTPB = 256;              // threads per block for every kernel
+constexpr int kWaves = kTPB / 64;
+constexpr int kMaxProbes = 256;        // probe slots per chunk pass
+constexpr int kPermMaxS = 16;
+
+// Permutation launch description (passed by value).
+struct PermArgs {
+    int32_t n, s, a, b;                 // tile bits, input-/output-contiguous low bits
+    int32_t ncomp, pad[3];
+    int32_t sin_hi[kPermMaxS];          // r bit a+k -> input position
+    int32_t sout_hi[kPermMaxS];         // w bit b+k -> output position
+    int32_t w2r[kPermMaxS];             // w bit k -> r bit
+    int32_t comp_in[64];                // block-index bit m -> input position
+    int32_t comp_out[64];               // ... -> output position
+};
+
+// Pauli term on a chunk: coef * i^{|x&z|} X^x Z^z  (Y where x and z both set)
+struct PauliDev {
+    uint64_t xmask, zmask;     // state bit masks
+    int32_t xloc, pad;         // x mask in chunk-local bits
+    double cre, cim;           // coefficient (complex)
+};
+
+enum PauliMode : int32_t {
+    PAULI_WRITE = 1,   // lam = sum of terms  (overwrite)
+    PAULI_ACC = 2,     // lam += sum of terms
+    PAULI_EXPECT = 4,  // partial[block] = sum conj(psi) * (terms psi)
+};
+
+// Launchers (qsim_kernels.hip).  All take the stream last and return the
+// launch status.
+template <typename R>
+hipError_t launch_init_basis(cx<R> *psi, uint64_t dim, uint64_t index, hipStream_t s);
+template <typename R>
+hipError_t launch_1q(cx<R> *psi, int n, int q, const Mat1<R> &U, hipStream_t s);
+template <typename R>
+hipError_t launch_2q(cx<R> *psi, int n, int q0, int q1, const Mat2<R> &U, hipStream_t s);
+template <typename R>
+hipError_t launch_diag1q(cx<R> *psi, int n, int q, cx<R> p0, cx<R> p1, hipStream_t s);
+template <typename R>
+hipError_t launch_permute(const cx<R> *in, cx<R> *out, const PermArgs &a, hipStream_t s);
+template <typename R>
+hipError_t launch_tile_pass(cx<R> *psi, cx<R> *lam, int n, const PassDev &host_pass,
+                            const PassDev *d_pass, const StepDev *d_steps, const cx<R> *d_mats,
+                            double *d_probe_partial, hipStream_t s);
+hipError_t launch_probe_reduce(const double *partial, int nblocks, int nprobe,
+                               const int32_t *d_ids, double *out, hipStream_t s);
+template <typename R>
+hipError_t launch_pauli_chunk(const cx<R> *psi, cx<R> *lam, int n, const PassDev &host_pass,
+                              const PassDev *d_pass, const PauliDev *d_terms, int nterms,
+                              int mode, double *d_partial, hipStream_t s);
+template <typename R>
+hipError_t launch_dot(const cx<R> *a, const cx<R> *b, uint64_t dim, double *d_partial,
+                      int *nblocks_out, hipStream_t s);
+hipError_t launch_sum_partials(const double *partial, int nblocks, int width, double *out,
+                               hipStream_t s);
+template <typename R>
+hipError_t launch_scale(cx<R> *psi, uint64_t dim, double factor, hipStream_t s);
+
+int max_dot_blocks();
+hipError_t prepare_kernels();
+
+}  // namespace qsim

@@ -1,0 +1,4 @@
+from .passes import LazyQubitReordering, Tile
+from .pm import PassManager, make_custom_pm
+
+__all__ = ["LazyQubitReordering", "Tile", "PassManager", "make_custom_pm"]

@@ -1,0 +1,95 @@
+"""Circuit generators for the benchmark workloads.
+
+* ``random_circuit(n, depth, seed)``: per layer one gate drawn uniformly from
+  {H, X, Y, Z, S, T, RX, RY, RZ} on every qubit (angles U[0, 2pi)), then CX
+  (control first) on the pairs (perm[2i], perm[2i+1]) of a seeded permutation,
+  each with probability 0.5.  RNG: ``numpy.random.Generator(PCG64(seed))``.
+* ``qft_circuit(n)``: H and CP(pi/2^k) ladder, then the n/2 final SWAPs; the
+  CP angles are computed in FP64.
+* ``ansatz_circuit(n, layers, theta)``: RY(theta), RZ(theta) on every qubit
+  (parameters interleaved per qubit), then CZ on the ring (i, i+1 mod n).
+* ``ring_observable(n)``: H = sum_i Z_i Z_{i+1 mod n} + 0.5 sum_i X_i as
+  (xmask, zmask, coef) terms.
+"""
+import math
+
+import numpy as np
+
+from .circuit import QuantumCircuit
+
+RANDOM_GATESET = ("h", "x", "y", "z", "s", "t", "rx", "ry", "rz")
+
+
+def random_circuit(n, depth, seed, measure_marker=False):
+    rng = np.random.Generator(np.random.PCG64(seed))
+    qc = QuantumCircuit(n)
+    for _ in range(depth):
+        for q in range(n):
+            name = RANDOM_GATESET[int(rng.integers(0, len(RANDOM_GATESET)))]
+            if name in ("rx", "ry", "rz"):
+                getattr(qc, name)(float(rng.uniform(0.0, 2 * math.pi)), q)
+            else:
+                getattr(qc, name)(q)
+        perm = rng.permutation(n)
+        for i in range(n // 2):
+            if rng.random() < 0.5:
+                qc.cx(int(perm[2 * i]), int(perm[2 * i + 1]))
+    if measure_marker:
+        qc.save_statevector()
+    return qc
+
+
+def qft_circuit(n, swaps=True):
+    qc = QuantumCircuit(n)
+    for j in reversed(range(n)):
+        qc.h(j)
+        for k in reversed(range(j)):
+            qc.cp(math.pi / float(2 ** (j - k)), k, j)
+    if swaps:
+        for i in range(n // 2):
+            qc.swap(i, n - 1 - i)
+    return qc
+
+
+def ansatz_params(n, layers, seed=1003):
+    rng = np.random.Generator(np.random.PCG64(seed))
+    return rng.uniform(-math.pi, math.pi, size=layers * 2 * n)
+
+
+def ansatz_spec(n, layers):
+    """Parameterised op list: (name, qubits, param_index or None)."""
+    spec = []
+    k = 0
+    for _ in range(layers):
+        for q in range(n):
+            spec.append(("ry", (q,), k))
+            spec.append(("rz", (q,), k + 1))
+            k += 2
+        for q in range(n):
+            spec.append(("cz", (q, (q + 1) % n), None))
+    return spec
+
+
+def ansatz_circuit(n, layers, theta):
+    qc = QuantumCircuit(n)
+    for name, qs, k in ansatz_spec(n, layers):
+        if k is None:
+            getattr(qc, name)(*qs)
+        else:
+            getattr(qc, name)(float(theta[k]), *qs)
+    return qc
+
+
+def ring_observable(n):
+    terms = []
+    for i in range(n):
+        terms.append((0, (1 << i) | (1 << ((i + 1) % n)), 1.0))
+    for i in range(n):
+        terms.append((1 << i, 0, 0.5))
+    return terms
+
+
+def random_state(n, seed):
+    rng = np.random.Generator(np.random.PCG64(seed))
+    psi = rng.standard_normal(1 << n) + 1j * rng.standard_normal(1 << n)
+    return psi / np.linalg.norm(psi)

@@ -1,0 +1,165 @@
+/*
+ * qgpusim -- C ABI of the MI355X (gfx950) statevector backend.
+ *
+ * This is the drop-in boundary that replaces the reference's pybind11 module
+ * `qgpusim_cuda` (qgpusim/backends/custom/qgpusim_cuda/src/bindings.cpp:1004-1092).
+ * Every entry point returns 0 on success and a non-zero status on failure;
+ * qsim_last_error() then holds the message (the text the reference raised as
+ * RuntimeError).  Entry points that take a qsim_state run asynchronously on the
+ * state's own HIP stream unless stated otherwise.  A state is not thread-safe:
+ * use it from one thread at a time.
+ *
+ * Amplitude layout: index i = sum_q b_q 2^q, complex64 (interleaved float re,
+ * im) or complex128 (interleaved double).  Matrices are row-major in the
+ * state's precision; a 4x4 matrix on (q0, q1) is little-endian: row/column
+ * index b_q0 + 2*b_q1 (kernels.cu:54-55).
+ */
+#ifndef QGPUSIM_H
+#define QGPUSIM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { QSIM_C64 = 0, QSIM_C128 = 1 };
+
+/* tile_mode of run_custom_backend (qgpusim/backends/custom/backend.py:460-473) */
+enum { QSIM_SEQUENTIAL = 0, QSIM_GRAPHED = 1, QSIM_COOPERATIVE = 2 };
+
+enum { QSIM_OP_1Q = 1, QSIM_OP_2Q = 2, QSIM_OP_PROBE = 3 };
+
+typedef struct qsim_state qsim_state;
+
+/* One op of a tile.  Replaces the ('1q', q0, U_dev) / ('2q', q0, q1, U_dev)
+ * tuples parsed at bindings.cpp:182-220: the matrix is `mat_offset` complex
+ * elements into the matrix arena passed beside the ops.  QSIM_OP_PROBE (only
+ * for qsim_adjoint) uses q1 as its output slot. */
+typedef struct {
+    int32_t kind, q0, q1, mat_offset;
+} qsim_op;
+
+/* Pauli term coef * i^{popcount(x&z)} X^x Z^z (Y where x and z both set). */
+typedef struct {
+    uint64_t xmask, zmask;
+    double coef_re, coef_im;
+} qsim_pauli;
+
+/* Message of the last failed call on this thread ("" if none). */
+const char *qsim_last_error(void);
+
+/* ABI version (incremented on incompatible changes). */
+int qsim_abi_version(void);
+
+/* Statevector(num_qubits) / StatevectorF32(num_qubits):
+ * bindings.cpp:92-110 and :645-658.  Allocates the amplitudes on the current
+ * HIP device and sets |0...0>. */
+int qsim_create(int num_qubits, int dtype, qsim_state **out);
+
+/* ~Statevector (bindings.cpp:112-122). */
+int qsim_destroy(qsim_state *st);
+
+/* num_qubits / dim properties (bindings.cpp:622-623). */
+int qsim_num_qubits(const qsim_state *st);
+uint64_t qsim_dim(const qsim_state *st);
+int qsim_dtype(const qsim_state *st);
+
+/* Device address of the amplitudes (changes after qsim_permute). */
+int qsim_device_ptr(qsim_state *st, void **out);
+
+/* reset() -> |0...0> (bindings.cpp:577-582). */
+int qsim_reset(qsim_state *st);
+
+/* from_numpy / to_numpy (bindings.cpp:547-574).  `count` is the number of
+ * amplitudes and must equal the state's dimension.  Both synchronise. */
+int qsim_from_host(qsim_state *st, const void *src, uint64_t count);
+int qsim_to_host(qsim_state *st, void *dst, uint64_t count);
+
+/* synchronize() (bindings.cpp:585-587): waits for the state's stream. */
+int qsim_synchronize(qsim_state *st);
+
+/* Host only, no device work: map[p_old] = new position of the qubit held at
+ * p_old, with the validation order and messages of bindings.cpp:435-464. */
+int qsim_permute_map(int num_qubits, const int32_t *layout_old, const int32_t *layout_new,
+                     int32_t *map_out);
+
+/* permute(layout_old, layout_new) (bindings.cpp:434-483): one LDS-tiled
+ * gather/scatter pass into the second buffer, then the buffers swap. */
+int qsim_permute(qsim_state *st, const int32_t *layout_old, const int32_t *layout_new, int num_qubits);
+
+/* Host only: the integer checks of parse_tile_ops (bindings.cpp:196-216). */
+int qsim_validate_ops(int num_qubits, const qsim_op *ops, int nops, uint64_t mat_count);
+
+/* apply_tile_dev / apply_tile_graphed / apply_tile_cooperative
+ * (bindings.cpp:223-240, 244-306, 312-373).
+ *   QSIM_SEQUENTIAL : one coalesced 1q/2q stream kernel per op;
+ *   QSIM_COOPERATIVE: the ops are planned into chunk passes; each pass stages
+ *                     2^K-amplitude chunks in LDS and runs all its ops there;
+ *   QSIM_GRAPHED    : the chunk passes captured into a hipGraph, cached on the
+ *                     (buffer, ops, matrices) signature and replayed on a hit.
+ * `mats` holds `mat_count` complex elements in the state's precision. */
+int qsim_apply_ops(qsim_state *st, int mode, const qsim_op *ops, int nops, const void *mats,
+                   uint64_t mat_count);
+
+/* apply_1q / apply_2q with a host matrix (bindings.cpp:499-544). */
+int qsim_apply_1q(qsim_state *st, int target_qubit, const void *U);
+int qsim_apply_2q(qsim_state *st, int q0, int q1, const void *U);
+
+/* apply_diagonal_1q (bindings.cpp:611-619). */
+int qsim_apply_diagonal_1q(qsim_state *st, int target_qubit, double p0_re, double p0_im,
+                           double p1_re, double p1_im);
+
+/* Module functions apply_1q_gate / apply_2q_gate on a caller-owned device
+ * vector (bindings.cpp:931-1001).  `stream` may be NULL (default stream).
+ * The call returns once the kernel has completed. */
+int qsim_apply_1q_raw(void *psi, int dtype, int num_qubits, int target_qubit, const void *U,
+                      void *stream);
+int qsim_apply_2q_raw(void *psi, int dtype, int num_qubits, int q0, int q1, const void *U,
+                      void *stream);
+
+/* ---- No counterpart in the reference: required by the benchmark workloads. ---- */
+
+/* <psi|psi> in FP64 (Neumaier-compensated, fixed order).  Synchronises. */
+int qsim_norm2(qsim_state *st, double *out);
+
+/* <a|b> in FP64.  Both states need the same size and precision.  Synchronises. */
+int qsim_overlap(qsim_state *a, qsim_state *b, double *re, double *im);
+
+/* Scale every amplitude by `factor` (periodic renormalisation). */
+int qsim_scale(qsim_state *st, double factor);
+
+/* <psi| sum_t P_t |psi> in FP64 over chunk passes.  Synchronises. */
+int qsim_expect_pauli(qsim_state *st, const qsim_pauli *terms, int nterms, double *re, double *im);
+
+/* Adjoint gradient.  lam = H psi (with <psi|H|psi> in FP64), then the reverse
+ * op list (inverse gates, each parameterised gate preceded by a
+ * QSIM_OP_PROBE carrying its generator G) runs over (psi, lam) in chunk
+ * passes; probe slot k receives Im<lam|G|psi> at its position, reduced in
+ * FP64.  psi is left at the circuit's input state.  Uses the state's second
+ * buffer for lam.  Synchronises. */
+int qsim_adjoint(qsim_state *st, const qsim_pauli *terms, int nterms, const qsim_op *rev_ops,
+                 int nops, const void *mats, uint64_t mat_count, int nprobes, double *probes_out,
+                 double *expect_out);
+
+/* Chunk-pass geometry: K chunk bits, cmin always-present low bits. */
+int qsim_set_tile_params(qsim_state *st, int K, int cmin);
+
+/* Counters: [0] kernel launches, [1] graph launches, [2] graph captures,
+ * [3] chunk passes, [4] permutations.  Returns the number written. */
+int qsim_get_counters(const qsim_state *st, int64_t *out, int n);
+
+/* Host only: plan `ops` into chunk passes without a device.  Writes, per pass,
+ * [K, cfast, nsteps, pos[0..K-1]] into `desc` (capacity desc_cap ints) and the
+ * op order of execution into `order` (nops entries; each op index once).
+ * Returns the number of passes, or -1. */
+int qsim_plan_describe(int num_qubits, int dtype, const qsim_op *ops, int nops, const void *mats,
+                       uint64_t mat_count, int K, int cmin, int32_t *desc, int desc_cap,
+                       int32_t *order);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* QGPUSIM_H */
