@@ -55,6 +55,7 @@ _SIGNATURES = {
     "qsim_from_host": ([_P, _P, _U64], _I),
     "qsim_to_host": ([_P, _P, _U64], _I),
     "qsim_synchronize": ([_P], _I),
+    "qsim_get_amplitude": ([_P, _U64, ctypes.POINTER(_D), ctypes.POINTER(_D)], _I),
     "qsim_permute_map": ([_I, _P, _P, _P], _I),
     "qsim_permute": ([_P, _P, _P, _I], _I),
     "qsim_validate_ops": ([_I, _P, _I, _U64], _I),
@@ -71,6 +72,8 @@ _SIGNATURES = {
     "qsim_adjoint": ([_P, _P, _I, _P, _I, _P, _U64, _I, _P, ctypes.POINTER(_D)], _I),
     "qsim_set_tile_params": ([_P, _I, _I], _I),
     "qsim_get_counters": ([_P, _P, _I], _I),
+    "qsim_profile": ([_P, _I], _I),
+    "qsim_profile_read": ([_P, _P, _P, _I], _I),
     "qsim_plan_describe": ([_I, _I, _P, _I, _P, _U64, _I, _I, _P, _I, _P], _I),
 }
 
@@ -211,6 +214,11 @@ class _StatevectorBase:
     def synchronize(self):
         _check(lib.qsim_synchronize(self._h))
 
+    def amplitude(self, index):
+        re, im = _D(), _D()
+        _check(lib.qsim_get_amplitude(self._h, int(index), ctypes.byref(re), ctypes.byref(im)))
+        return complex(re.value, im.value)
+
     def device_ptr(self):
         p = _P()
         _check(lib.qsim_device_ptr(self._h, ctypes.byref(p)))
@@ -311,6 +319,20 @@ class _StatevectorBase:
 
     def set_tile_params(self, K, cmin):
         _check(lib.qsim_set_tile_params(self._h, int(K), int(cmin)))
+
+    def profile(self, enable=True):
+        """Time each kernel launch with HIP events on this state's stream."""
+        _check(lib.qsim_profile(self._h, 1 if enable else 0))
+
+    def profile_read(self):
+        """{class: (total_ms, launches)} since the last read (synchronises)."""
+        ms = np.zeros(4, dtype=np.float64)
+        cnt = np.zeros(4, dtype=np.int64)
+        m = lib.qsim_profile_read(self._h, ms.ctypes.data, cnt.ctypes.data, 4)
+        if m < 0:
+            raise RuntimeError(lib.qsim_last_error().decode())
+        names = ("chunk_pass", "permute", "gate_stream", "pauli_chunk")
+        return {names[i]: (float(ms[i]), int(cnt[i])) for i in range(m)}
 
     def counters(self):
         buf = (ctypes.c_int64 * 5)()

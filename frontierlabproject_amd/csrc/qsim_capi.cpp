@@ -122,11 +122,54 @@ struct qsim_state {
     double *d_probe = nullptr;     // probe partials
     size_t probe_cap = 0;          // doubles
     int64_t counters[5] = {0, 0, 0, 0, 0};
+    // per-kernel-class timing with events on `stream` (off by default)
+    bool prof = false, capturing = false;
+    struct Rec {
+        int cls;
+        hipEvent_t a, b;
+    };
+    std::vector<Rec> recs;
+    std::vector<hipEvent_t> pool;
 };
 
 namespace {
 
 constexpr size_t kMaxGraphs = 64;
+constexpr int kProfClasses = 4;  // 0 chunk pass, 1 permutation, 2 gate stream, 3 Pauli chunk
+
+hipEvent_t take_event(qsim_state *st)
+{
+    if (!st->pool.empty()) {
+        hipEvent_t e = st->pool.back();
+        st->pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+// Brackets the launches issued during its lifetime with two events.
+struct ProfScope {
+    qsim_state *st;
+    int cls;
+    hipEvent_t a = nullptr;
+    ProfScope(qsim_state *s, int c) : st(s), cls(c)
+    {
+        if (st->prof && !st->capturing) {
+            a = take_event(st);
+            if (a) hipEventRecord(a, st->stream);
+        }
+    }
+    ~ProfScope()
+    {
+        if (!a) return;
+        hipEvent_t b = take_event(st);
+        if (!b) return;
+        hipEventRecord(b, st->stream);
+        st->recs.push_back({cls, a, b});
+    }
+};
 
 int ensure_tmp(qsim_state *st)
 {
@@ -209,6 +252,7 @@ int run_sequential(qsim_state *st, const qsim_op *ops, int nops, const void *mat
     cx<R> *psi = static_cast<cx<R> *>(st->psi);
     for (int i = 0; i < nops; ++i) {
         const qsim_op &o = ops[i];
+        ProfScope ps(st, 2);
         if (o.kind == QSIM_OP_1Q) {
             Mat1<R> U;
             std::memcpy(U.u, m + o.mat_offset, sizeof(U));
@@ -235,6 +279,7 @@ int enqueue_passes(qsim_state *st, const Plan &plan, const DescImage &img, char 
     for (size_t p = 0; p < plan.passes.size(); ++p) {
         const PassDev &hp = plan.passes[p];
         const int nblocks = 1 << (st->n - hp.K);
+        ProfScope ps(st, 0);
         HIP_TRY(launch_tile_pass<R>(static_cast<cx<R> *>(st->psi), static_cast<cx<R> *>(lam), st->n, hp, dp + p, ds,
                                     dm, st->d_probe, st->stream));
         st->counters[0]++;
@@ -303,7 +348,9 @@ int run_graphed(qsim_state *st, const qsim_op *ops, int nops, const void *mats, 
     HIP_TRY(hipMemcpyAsync(e.desc, img.bytes.data(), img.bytes.size(), hipMemcpyHostToDevice, st->stream));
     HIP_TRY(hipStreamSynchronize(st->stream));
     HIP_TRY(hipStreamBeginCapture(st->stream, hipStreamCaptureModeThreadLocal));
+    st->capturing = true;
     int rc = enqueue_passes<R>(st, plan, img, static_cast<char *>(e.desc), nullptr, nullptr);
+    st->capturing = false;
     hipGraph_t g = nullptr;
     hipError_t ec = hipStreamEndCapture(st->stream, &g);
     if (rc || ec != hipSuccess) {
@@ -464,6 +511,7 @@ int pauli_t(qsim_state *st, const qsim_pauli *terms, int nterms, void *lam, doub
         if (lam) mode |= (p == 0 ? PAULI_WRITE : PAULI_ACC);
         const int nblocks = 1 << (st->n - pp.pd.K);
         char *base = static_cast<char *>(dev);
+        ProfScope ps(st, 3);
         HIP_TRY(launch_pauli_chunk<R>(static_cast<const cx<R> *>(st->psi), static_cast<cx<R> *>(lam), st->n, pp.pd,
                                       reinterpret_cast<const PassDev *>(base + pass_off[p]),
                                       reinterpret_cast<const PauliDev *>(base + term_off[p]), (int)pp.terms.size(),
@@ -569,6 +617,11 @@ int qsim_destroy(qsim_state *st)
     if (st->tmp) hipFree(st->tmp);
     if (st->d_small) hipFree(st->d_small);
     if (st->d_probe) hipFree(st->d_probe);
+    for (auto &r : st->recs) {
+        hipEventDestroy(r.a);
+        hipEventDestroy(r.b);
+    }
+    for (auto e : st->pool) hipEventDestroy(e);
     if (st->stream) hipStreamDestroy(st->stream);
     delete st;
     return 0;
@@ -614,6 +667,26 @@ int qsim_to_host(qsim_state *st, void *dst, uint64_t count)
     return 0;
 }
 
+int qsim_get_amplitude(qsim_state *st, uint64_t index, double *re, double *im)
+{
+    if (!st || !re || !im) return fail("null argument");
+    if (index >= st->dim) return fail("amplitude index out of range");
+    if (st->dtype == QSIM_C64) {
+        float h[2];
+        HIP_TRY(hipMemcpyAsync(h, static_cast<char *>(st->psi) + 8 * index, 8, hipMemcpyDeviceToHost, st->stream));
+        HIP_TRY(hipStreamSynchronize(st->stream));
+        *re = h[0];
+        *im = h[1];
+    } else {
+        double h[2];
+        HIP_TRY(hipMemcpyAsync(h, static_cast<char *>(st->psi) + 16 * index, 16, hipMemcpyDeviceToHost, st->stream));
+        HIP_TRY(hipStreamSynchronize(st->stream));
+        *re = h[0];
+        *im = h[1];
+    }
+    return 0;
+}
+
 int qsim_synchronize(qsim_state *st)
 {
     if (!st) return fail("null state");
@@ -640,6 +713,7 @@ int qsim_permute(qsim_state *st, const int32_t *layout_old, const int32_t *layou
     if (ensure_tmp(st)) return -1;
     PermArgs P;
     build_perm_args(st->n, map, st->dtype == QSIM_C64 ? 12 : 11, P);
+    ProfScope ps(st, 1);
     if (st->dtype == QSIM_C64)
         HIP_TRY(launch_permute<float>(static_cast<cx<float> *>(st->psi), static_cast<cx<float> *>(st->tmp), P,
                                       st->stream));
@@ -817,6 +891,36 @@ int qsim_set_tile_params(qsim_state *st, int K, int cmin)
     st->K = K;
     st->cmin = cmin;
     return 0;
+}
+
+int qsim_profile(qsim_state *st, int enable)
+{
+    if (!st) return fail("null state");
+    st->prof = enable != 0;
+    return 0;
+}
+
+int qsim_profile_read(qsim_state *st, double *ms_out, int64_t *count_out, int nclasses)
+{
+    if (!st || !ms_out || !count_out) return fail("null argument");
+    HIP_TRY(hipStreamSynchronize(st->stream));
+    const int m = std::min(nclasses, kProfClasses);
+    for (int c = 0; c < m; ++c) {
+        ms_out[c] = 0.0;
+        count_out[c] = 0;
+    }
+    for (auto &r : st->recs) {
+        float ms = 0.f;
+        HIP_TRY(hipEventElapsedTime(&ms, r.a, r.b));
+        if (r.cls < m) {
+            ms_out[r.cls] += ms;
+            count_out[r.cls] += 1;
+        }
+        st->pool.push_back(r.a);
+        st->pool.push_back(r.b);
+    }
+    st->recs.clear();
+    return m;
 }
 
 int qsim_get_counters(const qsim_state *st, int64_t *out, int n)

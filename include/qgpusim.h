@@ -77,6 +77,10 @@ int qsim_reset(qsim_state *st);
 int qsim_from_host(qsim_state *st, const void *src, uint64_t count);
 int qsim_to_host(qsim_state *st, void *dst, uint64_t count);
 
+/* One amplitude, copied to the host after the state's pending work.  No
+ * reference counterpart (used for fidelity checks without a full copy). */
+int qsim_get_amplitude(qsim_state *st, uint64_t index, double *re, double *im);
+
 /* synchronize() (bindings.cpp:585-587): waits for the state's stream. */
 int qsim_synchronize(qsim_state *st);
 
@@ -145,6 +149,15 @@ int qsim_adjoint(qsim_state *st, const qsim_pauli *terms, int nterms, const qsim
 
 /* Chunk-pass geometry: K chunk bits, cmin always-present low bits. */
 int qsim_set_tile_params(qsim_state *st, int K, int cmin);
+
+/* Per-kernel-class timing with HIP events recorded on the state's stream around
+ * each launch (not inside captured graphs).  Classes: 0 chunk pass,
+ * 1 permutation, 2 gate stream, 3 Pauli chunk pass. */
+int qsim_profile(qsim_state *st, int enable);
+
+/* Synchronises, then writes the summed milliseconds and launch counts per class
+ * recorded since the last read; returns the number of classes written. */
+int qsim_profile_read(qsim_state *st, double *ms_out, int64_t *count_out, int nclasses);
 
 /* Counters: [0] kernel launches, [1] graph launches, [2] graph captures,
  * [3] chunk passes, [4] permutations.  Returns the number written. */
