@@ -20,7 +20,8 @@ import os
 import numpy as np
 
 _PKG = os.path.dirname(os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__)))))
-LIB_PATH = os.path.join(_PKG, "lib", "libqgpusim_hip.so")
+# QGPUSIM_LIB selects another build of the same library (e.g. a tuning variant)
+LIB_PATH = os.environ.get("QGPUSIM_LIB") or os.path.join(_PKG, "lib", "libqgpusim_hip.so")
 
 C64, C128 = 0, 1
 SEQUENTIAL, GRAPHED, COOPERATIVE = 0, 1, 2
@@ -416,10 +417,11 @@ def validate_ops(ops, num_qubits):
     _check(lib.qsim_validate_ops(num_qubits, ctypes.addressof(arr), len(ops), arena.size))
 
 
-def plan_describe(num_qubits, arr, nops, arena, K, cmin):
-    """Host-only chunk plan: ([(K, cfast, nsteps, pos)], execution order)."""
+def plan_describe(num_qubits, arr, nops, arena, K, cmin, steps=False):
+    """Host-only chunk plan: ([(K, cfast, nsteps, pos[, steps])], execution order),
+    a step being (register bits, rounds, dense)."""
     code = C128 if arena.dtype == np.complex128 else C64
-    cap = 4096 + 64 * max(1, nops)
+    cap = 4096 + 128 * max(1, nops)
     desc = np.zeros(cap, dtype=np.int32)
     order = np.zeros(max(1, nops), dtype=np.int32)
     npass = lib.qsim_plan_describe(num_qubits, code, ctypes.addressof(arr), nops, arena.ctypes.data,
@@ -430,6 +432,11 @@ def plan_describe(num_qubits, arr, nops, arena, K, cmin):
     w = 0
     for _ in range(npass):
         k, cf, ns = int(desc[w]), int(desc[w + 1]), int(desc[w + 2])
-        passes.append((k, cf, ns, desc[w + 3:w + 3 + k].tolist()))
+        pos = desc[w + 3:w + 3 + k].tolist()
         w += 3 + k
+        st = []
+        for _s in range(ns):
+            st.append((desc[w:w + 4].tolist(), int(desc[w + 4]), bool(desc[w + 5])))
+            w += 6
+        passes.append((k, cf, ns, pos, st) if steps else (k, cf, ns, pos))
     return passes, order[:nops].tolist()

@@ -26,9 +26,10 @@ def hipcc():
     raise RuntimeError("hipcc not found: the HIP toolchain is required to build the backend")
 
 
-def build(verbose=False, extra_flags=()):
-    os.makedirs(LIBDIR, exist_ok=True)
-    tmp = LIB + ".tmp%d" % os.getpid()
+def build(verbose=False, extra_flags=(), out=None):
+    out = out or LIB
+    os.makedirs(os.path.dirname(out), exist_ok=True)
+    tmp = out + ".tmp%d" % os.getpid()
     cmd = [hipcc(), f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-shared",
            "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
            "-Wno-unused-command-line-argument",
@@ -40,9 +41,9 @@ def build(verbose=False, extra_flags=()):
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         sys.stderr.write(res.stdout + res.stderr)
-        raise RuntimeError("hipcc failed building %s" % LIB)
-    os.replace(tmp, LIB)
-    return LIB
+        raise RuntimeError("hipcc failed building %s" % out)
+    os.replace(tmp, out)
+    return out
 
 
 if __name__ == "__main__":

@@ -943,11 +943,17 @@ int qsim_plan_describe(int num_qubits, int dtype, const qsim_op *ops, int nops, 
         return fail(err);
     int w = 0;
     for (const PassDev &p : plan.passes) {
-        if (w + 3 + p.K > desc_cap) return fail("plan_describe: descriptor buffer too small");
+        if (w + 3 + p.K + 6 * p.nsteps > desc_cap) return fail("plan_describe: descriptor buffer too small");
         desc[w++] = p.K;
         desc[w++] = p.cfast;
         desc[w++] = p.nsteps;
         for (int k = 0; k < p.K; ++k) desc[w++] = p.pos[k];
+        for (int s = p.step_off; s < p.step_off + p.nsteps; ++s) {
+            const StepDev &sd = plan.steps[s];
+            for (int t = 0; t < kG; ++t) desc[w++] = sd.bits[t];
+            desc[w++] = sd.nrounds;
+            desc[w++] = sd.dense >= 0 ? 1 : 0;
+        }
     }
     for (size_t i = 0; i < plan.order.size(); ++i) order[i] = plan.order[i];
     return (int)plan.passes.size();

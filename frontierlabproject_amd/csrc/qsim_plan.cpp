@@ -2,6 +2,7 @@
 #include "qsim_plan.h"
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <utility>
 
@@ -27,6 +28,15 @@ Cd mat_at(const void *mats, int elem_bytes, int64_t idx)
 
 inline bool is0(Cd c) { return c.re == 0.0 && c.im == 0.0; }
 inline bool is1(Cd c) { return c.re == 1.0 && c.im == 0.0; }
+inline bool isre(Cd c) { return c.im == 0.0; }
+inline bool isim(Cd c) { return c.re == 0.0; }
+inline bool iseq(Cd c, double re, double im) { return c.re == re && c.im == im; }
+inline Cd cmulh(Cd a, Cd b) { return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re}; }
+inline Cd cdivh(Cd a, Cd b)
+{
+    const double d = b.re * b.re + b.im * b.im;
+    return {(a.re * b.re + a.im * b.im) / d, (a.im * b.re - a.re * b.im) / d};
+}
 
 uint64_t fill_low(uint64_t set, int width, int target)
 {
@@ -73,10 +83,16 @@ struct StepBuilder {
     int last[kG];          // last slot used per register bit
     StepDev sd;
     std::vector<std::pair<int, int>> placed;  // (slot * 2 + sub, op index)
+    bool dense = false;    // the step applies one fused 16x16 matrix
+    bool has_probe = false;
+    Cd factor = {1.0, 0.0};  // scalar taken out of this step's slot gates
+    double cost = 0.0;     // estimated multiply-adds per amplitude of the slot form
+    Cd M[256];             // dense matrix (row-major), valid when dense
 
     StepBuilder()
     {
         std::memset(&sd, 0, sizeof(sd));
+        sd.dense = -1;
         for (int t = 0; t < kG; ++t) last[t] = -1, regbit[t] = -1;
         for (int t = 0; t < k1qSlots; ++t) sd.probe[t] = -1;
     }
@@ -95,6 +111,8 @@ struct Ctx {
     Plan *plan;
     PassDev *pd;
     int max_probes;
+    const int *loc;      // state bit -> chunk-local bit
+    bool real_scale;     // non-unit scalars may be factored out (no probes)
 };
 
 int push_mats(Plan &p, const Cd *v, int count)
@@ -108,7 +126,8 @@ int push_mats(Plan &p, const Cd *v, int count)
 }
 
 // Place op `idx` (register bits rx, ry already assigned) into the step.
-bool place(Ctx &c, StepBuilder &sb, int idx, int rx, int ry)
+// Returns 1 when placed, 0 when no slot is free, 2 when the op needs a dense step.
+int place(Ctx &c, StepBuilder &sb, int idx, int rx, int ry)
 {
     const PlanOp &op = c.ops[idx];
     Plan &p = *c.plan;
@@ -126,34 +145,77 @@ bool place(Ctx &c, StepBuilder &sb, int idx, int rx, int ry)
                 p.mats[2 * (off + 4 + k) + 1] = u[k].im;
             }
             sb.sd.kind[j] = S1_PROBE_GEN;
+            sb.cost += 8.0;
             sb.placed.push_back({2 * j + 1, idx});
-            return true;
+            return 1;
         }
         const int j = next_slot(sb.last[t], t);
-        if (j < 0) return false;
+        if (j < 0) return 0;
         if (op.kind == OP_PROBE) {
-            if (c.pd->nprobe >= c.max_probes) return false;
+            if (c.pd->nprobe >= c.max_probes) return 0;
             Cd g8[8] = {u[0], u[1], u[2], u[3], {1, 0}, {0, 0}, {0, 0}, {1, 0}};
             sb.sd.kind[j] = S1_PROBE;
+            sb.has_probe = true;
             sb.sd.mat[j] = push_mats(p, g8, 8);
             sb.sd.probe[(j / kSlotsPerRound) * 4 + t] = c.pd->nprobe++;
             p.probe_ids.push_back(op.probe);
+        } else if (is0(u[1]) && is0(u[2]) &&
+                   (c.real_scale ? !is0(u[0]) : std::fabs(u[0].re * u[0].re + u[0].im * u[0].im - 1.0) < 1e-12)) {
+            // diag(u0, u3) = u0 * diag(1, u3/u0): the phase u0 is applied later
+            const Cd d = cdivh(u[3], u[0]);
+            sb.factor = cmulh(sb.factor, u[0]);
+            if (iseq(d, -1, 0)) {
+                sb.sd.kind[j] = S1_Z;
+                sb.cost += 0.5;
+            } else if (iseq(d, 0, 1)) {
+                sb.sd.kind[j] = S1_PI;
+                sb.cost += 0.5;
+            } else if (iseq(d, 0, -1)) {
+                sb.sd.kind[j] = S1_MI;
+                sb.cost += 0.5;
+            } else {
+                sb.sd.kind[j] = S1_PHASE;
+                sb.sd.mat[j] = push_mats(p, &d, 1);
+                sb.cost += 2.0;
+            }
         } else if (is0(u[1]) && is0(u[2])) {
             Cd d[2] = {u[0], u[3]};
             sb.sd.kind[j] = S1_DIAG;
             sb.sd.mat[j] = push_mats(p, d, 2);
+            sb.cost += 4.0;
+        } else if (is0(u[0]) && is0(u[3]) && is1(u[1]) && is1(u[2])) {
+            sb.sd.kind[j] = S1_X;
+            sb.cost += 0.5;
+        } else if (is0(u[0]) && is0(u[3]) && iseq(u[1], 0, -1) && iseq(u[2], 0, 1)) {
+            sb.sd.kind[j] = S1_Y;
+            sb.cost += 1.0;
+        } else if (c.real_scale && isre(u[0]) && u[0].re != 0.0 && iseq(u[1], u[0].re, 0) &&
+                   iseq(u[2], u[0].re, 0) && iseq(u[3], -u[0].re, 0)) {
+            // s * [[1, 1], [1, -1]] (H): butterfly, s applied later
+            sb.factor = cmulh(sb.factor, u[0]);
+            sb.sd.kind[j] = S1_BFLY;
+            sb.cost += 2.0;
+        } else if (isre(u[0]) && isre(u[1]) && isre(u[2]) && isre(u[3])) {
+            Cd r[2] = {{u[0].re, u[1].re}, {u[2].re, u[3].re}};
+            sb.sd.kind[j] = S1_REAL;
+            sb.cost += 4.0;
+            sb.sd.mat[j] = push_mats(p, r, 2);
+        } else if (isre(u[0]) && isre(u[3]) && isim(u[1]) && isim(u[2])) {
+            Cd r[2] = {{u[0].re, u[1].im}, {u[2].im, u[3].re}};
+            sb.sd.kind[j] = S1_RI;
+            sb.cost += 4.0;
+            sb.sd.mat[j] = push_mats(p, r, 2);
         } else {
             sb.sd.kind[j] = S1_GEN;
+            sb.cost += 8.0;
             sb.sd.mat[j] = push_mats(p, u, 4);
         }
         sb.last[t] = j;
         sb.placed.push_back({2 * j, idx});
-        return true;
+        return 1;
     }
     // pair op: orient to (first, second) = (lower, higher register bit)
     const int lo = rx < ry ? rx : ry, hi = rx < ry ? ry : rx;
-    const int j = next_slot(std::max(sb.last[rx], sb.last[ry]), 4 + pair_index(lo, hi));
-    if (j < 0) return false;
     auto sw = [](int i) { return ((i & 1) << 1) | ((i >> 1) & 1); };
     Cd m[16];
     for (int r = 0; r < 4; ++r)
@@ -162,37 +224,93 @@ bool place(Ctx &c, StepBuilder &sb, int idx, int rx, int ry)
             const int rr = rx == lo ? r : sw(r), qq = rx == lo ? q : sw(q);
             m[4 * r + q] = mat_at(c.mats, c.eb, (int64_t)op.mat_off + 4 * rr + qq);
         }
-    bool diag = true;
-    for (int r = 0; r < 4; ++r)
-        for (int q = 0; q < 4; ++q)
-            if (r != q && !is0(m[4 * r + q])) diag = false;
-    // control on the first bit: indices with b_first = 0 ({0, 2}) untouched
-    bool cf = is1(m[0]) && is1(m[10]) && is0(m[2]) && is0(m[8]);
-    for (int a : {0, 2})
-        for (int b : {1, 3}) cf = cf && is0(m[4 * a + b]) && is0(m[4 * b + a]);
-    // control on the second bit: indices {0, 1} untouched
-    bool cs = is1(m[0]) && is1(m[5]) && is0(m[1]) && is0(m[4]);
-    for (int a : {0, 1})
-        for (int b : {2, 3}) cs = cs && is0(m[4 * a + b]) && is0(m[4 * b + a]);
-    if (diag) {
-        Cd d[4] = {m[0], m[5], m[10], m[15]};
-        sb.sd.kind[j] = S2_DIAG;
-        sb.sd.mat[j] = push_mats(p, d, 4);
-    } else if (cf) {
-        Cd u[4] = {m[5], m[7], m[13], m[15]};  // rows/cols 1, 3
-        sb.sd.kind[j] = S2_CTRL_FIRST;
-        sb.sd.mat[j] = push_mats(p, u, 4);
-    } else if (cs) {
-        Cd u[4] = {m[10], m[11], m[14], m[15]};  // rows/cols 2, 3
-        sb.sd.kind[j] = S2_CTRL_SECOND;
-        sb.sd.mat[j] = push_mats(p, u, 4);
-    } else {
-        sb.sd.kind[j] = S2_GEN;
-        sb.sd.mat[j] = push_mats(p, m, 16);
-    }
+    auto only = [&](std::initializer_list<int> ones, int skip) {
+        for (int k = 0; k < 16; ++k) {
+            if (k == skip) continue;
+            bool one = false;
+            for (int o : ones) one |= o == k;
+            if (one ? !is1(m[k]) : !is0(m[k])) return false;
+        }
+        return true;
+    };
+    int kind;
+    if (only({0, 5, 10}, 15)) kind = S2_CPHASE;                  // diag(1,1,1,d)
+    else if (only({0, 10, 7, 13}, -1)) kind = S2_CX_FIRST;       // rows 1<->3
+    else if (only({0, 5, 11, 14}, -1)) kind = S2_CX_SECOND;      // rows 2<->3
+    else if (only({0, 6, 9, 15}, -1)) kind = S2_SWAP;            // rows 1<->2
+    else return 2;                                                // needs a dense step
+    const int j = next_slot(std::max(sb.last[rx], sb.last[ry]), 4 + pair_index(lo, hi));
+    if (j < 0) return 0;
+    sb.sd.kind[j] = kind;
+    if (kind == S2_CPHASE) sb.sd.mat[j] = push_mats(p, &m[15], 1);
+    sb.cost += kind == S2_CPHASE ? 1.0 : 0.5;
     sb.last[rx] = sb.last[ry] = j;
     sb.placed.push_back({2 * j, idx});
-    return true;
+    return 1;
+}
+
+// 16x16 embedding of op `idx` on register bits (rx, ry): out = E * in.
+void embed(Ctx &c, int idx, int rx, int ry, Cd *E)
+{
+    const PlanOp &op = c.ops[idx];
+    for (int k = 0; k < 256; ++k) E[k] = {0.0, 0.0};
+    const bool two = op.kind == OP_2Q;
+    const int k = two ? 2 : 1;
+    const int mask = (1 << rx) | (two ? (1 << ry) : 0);
+    for (int col = 0; col < 16; ++col) {
+        const int ci = ((col >> rx) & 1) | (two ? ((col >> ry) & 1) << 1 : 0);
+        for (int ri = 0; ri < (1 << k); ++ri) {
+            int row = col & ~mask;
+            row |= (ri & 1) << rx;
+            if (two) row |= ((ri >> 1) & 1) << ry;
+            E[16 * row + col] = mat_at(c.mats, c.eb, (int64_t)op.mat_off + (1 << k) * ri + ci);
+        }
+    }
+}
+
+void matmul16(const Cd *A, const Cd *B, Cd *out)  // out = A * B
+{
+    for (int r = 0; r < 16; ++r)
+        for (int q = 0; q < 16; ++q) {
+            double re = 0, im = 0;
+            for (int k = 0; k < 16; ++k) {
+                const Cd a = A[16 * r + k], b = B[16 * k + q];
+                re += a.re * b.re - a.im * b.im;
+                im += a.re * b.im + a.im * b.re;
+            }
+            out[16 * r + q] = {re, im};
+        }
+}
+
+// Fold op `idx` into the dense matrix of the step.
+void fold_dense(Ctx &c, StepBuilder &sb, int idx, int rx, int ry)
+{
+    Cd E[256], T[256];
+    embed(c, idx, rx, ry, E);
+    matmul16(E, sb.M, T);
+    std::memcpy(sb.M, T, sizeof(T));
+    sb.placed.push_back({(int)sb.placed.size(), idx});
+}
+
+// Turn a slot step into a dense one: multiply its placed ops in slot order.
+void to_dense(Ctx &c, StepBuilder &sb)
+{
+    std::vector<std::pair<int, int>> seq = sb.placed;
+    std::sort(seq.begin(), seq.end());
+    for (int k = 0; k < 256; ++k) sb.M[k] = {(k % 17) == 0 ? 1.0 : 0.0, 0.0};
+    sb.placed.clear();
+    sb.dense = true;
+    sb.factor = {1.0, 0.0};
+    for (auto &pr : seq) {
+        const PlanOp &op = c.ops[pr.second];
+        int rx = -1, ry = -1;
+        for (int t = 0; t < kG; ++t) {
+            if (sb.regbit[t] == c.loc[op.q0]) rx = t;
+            if (op.kind == OP_2Q && sb.regbit[t] == c.loc[op.q1]) ry = t;
+        }
+        fold_dense(c, sb, pr.second, rx, ry);
+    }
+    for (int j = 0; j < kSlots; ++j) sb.sd.kind[j] = 0;
 }
 
 }  // namespace
@@ -238,6 +356,9 @@ bool plan_passes(int n, const PlanOp *ops, int nops, const void *mats, int elem_
     std::vector<int> rem(nops);
     for (int i = 0; i < nops; ++i) rem[i] = i;
     const uint64_t low = cmin ? ((1ull << cmin) - 1) : 0;
+    bool any_probe = false;
+    for (int i = 0; i < nops; ++i) any_probe |= ops[i].kind == OP_PROBE;
+    Cd pending = {1.0, 0.0};  // scalar factored out of finished steps, not yet applied
 
     while (!rem.empty()) {
         // ---- pass: chunk bit set ----
@@ -265,7 +386,7 @@ bool plan_passes(int n, const PlanOp *ops, int nops, const void *mats, int elem_
         pd.step_off = (int32_t)out.steps.size();
         pd.probe_off = (int32_t)out.probe_ids.size();
         pd.nprobe = 0;
-        Ctx ctx{ops, mats, elem_bytes, &out, &pd, max_probes};
+        Ctx ctx{ops, mats, elem_bytes, &out, &pd, max_probes, loc, !any_probe};
 
         // ---- register steps ----
         std::vector<int> srem = chosen;
@@ -305,7 +426,20 @@ bool plan_passes(int n, const PlanOp *ops, int nops, const void *mats, int elem_
                         ok = false;
                     }
                 }
-                if (ok) ok = place(ctx, sb, idx, rx, ry);
+                if (ok) {
+                    if (sb.dense) {
+                        ok = op.kind != OP_PROBE;
+                        if (ok) fold_dense(ctx, sb, idx, rx, ry);
+                    } else {
+                        const int r = place(ctx, sb, idx, rx, ry);
+                        if (r == 2 && !sb.has_probe) {
+                            to_dense(ctx, sb);
+                            fold_dense(ctx, sb, idx, rx, ry);
+                        } else {
+                            ok = r == 1;
+                        }
+                    }
+                }
                 if (!ok) {
                     sb = save;
                     out.mats.resize(mats_before);
@@ -315,6 +449,8 @@ bool plan_passes(int n, const PlanOp *ops, int nops, const void *mats, int elem_
                     sleft.push_back(idx);
                 }
             }
+            // a slot step whose arithmetic exceeds one dense product becomes dense
+            if (!sb.dense && !sb.has_probe && sb.cost > 64.0) to_dense(ctx, sb);
             // fill unused register bits with the lowest free chunk bits
             for (int b = 0; b < K && sb.nreg < kG; ++b)
                 if (sb.reg_of(b) < 0) sb.regbit[sb.nreg++] = b;
@@ -323,6 +459,23 @@ bool plan_passes(int n, const PlanOp *ops, int nops, const void *mats, int elem_
             std::memcpy(sorted_bits, sb.regbit, sizeof(sorted_bits));
             std::sort(sorted_bits, sorted_bits + kG);
             for (int t = 0; t < kG; ++t) sb.sd.sbits[t] = sorted_bits[t];
+            if (sb.dense) {
+                sb.sd.dense = push_mats(out, sb.M, 256);
+                sb.sd.nrounds = 0;
+            } else {
+                int used = -1;
+                for (int j = 0; j < kSlots; ++j)
+                    if (sb.sd.kind[j]) used = j;
+                sb.sd.nrounds = used / kSlotsPerRound + 1;
+            }
+            // keep every amplitude within 2^+-24 of its exact value
+            pending = cmulh(pending, sb.factor);
+            sb.sd.scale = -1;
+            const double mag2 = pending.re * pending.re + pending.im * pending.im;
+            if (mag2 > 0x1p48 || mag2 < 0x1p-48) {
+                sb.sd.scale = push_mats(out, &pending, 1);
+                pending = {1.0, 0.0};
+            }
             std::sort(sb.placed.begin(), sb.placed.end());
             for (auto &pr : sb.placed) out.order.push_back(pr.second);
             out.steps.push_back(sb.sd);
@@ -331,6 +484,18 @@ bool plan_passes(int n, const PlanOp *ops, int nops, const void *mats, int elem_
         pd.nsteps = (int32_t)out.steps.size() - pd.step_off;
         out.passes.push_back(pd);
         rem.swap(left);
+    }
+    // the remaining factor goes to the last step
+    if (!(pending.re == 1.0 && pending.im == 0.0) && !out.steps.empty()) {
+        StepDev &last = out.steps.back();
+        if (last.scale >= 0) {
+            Cd v = {out.mats[2 * last.scale], out.mats[2 * last.scale + 1]};
+            v = cmulh(v, pending);
+            out.mats[2 * last.scale] = v.re;
+            out.mats[2 * last.scale + 1] = v.im;
+        } else {
+            last.scale = push_mats(out, &pending, 1);
+        }
     }
     return true;
 }

@@ -164,7 +164,8 @@ int qsim_profile_read(qsim_state *st, double *ms_out, int64_t *count_out, int nc
 int qsim_get_counters(const qsim_state *st, int64_t *out, int n);
 
 /* Host only: plan `ops` into chunk passes without a device.  Writes, per pass,
- * [K, cfast, nsteps, pos[0..K-1]] into `desc` (capacity desc_cap ints) and the
+ * [K, cfast, nsteps, pos[0..K-1]] followed, per step, by [register bits 0..3,
+ * rounds, dense] into `desc` (capacity desc_cap ints), and the
  * op order of execution into `order` (nops entries; each op index once).
  * Returns the number of passes, or -1. */
 int qsim_plan_describe(int num_qubits, int dtype, const qsim_op *ops, int nops, const void *mats,

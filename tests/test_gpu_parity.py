@@ -32,10 +32,15 @@ def _tuples(ops):
     return out
 
 
+AMP_TOL = {np.complex64: 2e-5, np.complex128: 1e-12}
+
+
 def _check(ref, got, dtype):
     f = dense.fidelity(ref, got)
     assert 1 - f <= TOL[dtype], (1 - f, dense.max_abs_err_phase_aligned(ref, got))
     assert abs(np.vdot(got, got).real - 1) < 100 * TOL[dtype] ** 0.5
+    # amplitudes themselves, global phase included
+    assert np.max(np.abs(np.asarray(ref) - got)) < AMP_TOL[dtype] * max(1.0, np.max(np.abs(ref)))
 
 
 @pytest.mark.parametrize("dtype", [np.complex64, np.complex128])
@@ -173,6 +178,29 @@ def test_config0_random_20q_c128_nl10():
         f = dense.fidelity(ref, psi)
         print(f"config0 mode={mode} 1-F={1 - f:.3e} maxabs={dense.max_abs_err_phase_aligned(ref, psi):.3e}")
         assert 1 - f <= 1e-10
+
+
+@pytest.mark.parametrize("dtype", [np.complex64, np.complex128])
+def test_deep_factored_gates(hip, dtype):
+    """Hundreds of H / RZ / T / S / Z gates in one tile: their scalars are
+    factored out and re-applied; the amplitudes must come out unscaled."""
+    n = 12
+    rng = np.random.default_rng(11)
+    ops = []
+    for _ in range(40):
+        for q in range(n):
+            name = ["h", "rz", "t", "s", "z", "h"][int(rng.integers(0, 6))]
+            U = G.rz(rng.uniform(0, 6)) if name == "rz" else G.matrix(name)
+            ops.append(("1q", q, U))
+    psi0 = C.random_state(n, 3)
+    ref = psi0
+    for o in ops:
+        ref = dense.apply(ref, o[2], [o[1]], n)
+    for mode in ("apply_tile_cooperative", "apply_tile_graphed"):
+        sv = _sv(hip, n, dtype)
+        sv.from_numpy(psi0)
+        getattr(sv, mode)(ops)
+        _check(ref, sv.to_numpy(), dtype)
 
 
 def test_graph_replay_hits(hip):
