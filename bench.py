@@ -54,6 +54,9 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--adj-n", type=int, default=24)
     ap.add_argument("--adj-layers", type=int, default=20)
+    ap.add_argument("--calib-gib", type=float, default=0.0,
+                    help="first copy this many GiB device-to-device (a known-size kernel for "
+                         "calibrating hardware byte counters)")
     return ap.parse_args()
 
 
@@ -94,6 +97,16 @@ def build_program(hip, n, depth, nL, seed):
         U = op.to_matrix()
         plain.append(("1q", idx[0], U) if len(idx) == 1 else ("2q", idx[0], idx[1], U))
     return qc, plan, prog, plain, ngates, t_transpile
+
+
+def measured_traffic(n, depth):
+    """HBM bytes per launch recorded from rocprofv3 --pmc runs of this workload
+    (profiles/traffic_random33_c64.json), or None for other workloads."""
+    path = os.path.join(ROOT, "profiles", "traffic_random33_c64.json")
+    if n != 33 or depth != 30 or not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f)["kernels"]
 
 
 def run_program(hip, sv, prog, mode):
@@ -139,6 +152,16 @@ def main():
     import numpy as np
     import torch
     from frontierlabproject_amd.backends.custom.native import qgpusim_hip as hip
+
+    if args.calib_gib > 0:
+        nel = int(args.calib_gib * (1 << 30)) // 4
+        src = torch.ones(nel, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"calibration copy: {args.calib_gib} GiB read + {args.calib_gib} GiB written")
+        del src, dst
+        torch.cuda.empty_cache()
 
     n = args.n
     bytes_per_pass = 2 * (1 << n) * 8
@@ -263,6 +286,7 @@ def main():
         log(f"cpu baseline: {cpu['value']} GB/s on {cpu['cores']} threads")
         extra["cpu_s_per_circuit_extrapolated"] = round(cpu.pop("s_per_gate") * ngates * 2 ** (n - 26), 1)
 
+    traffic = measured_traffic(n, args.depth)
     out = {
         "metric": METRIC,
         "value": round(value, 1),
@@ -283,10 +307,12 @@ def main():
                    "hbm_passes_per_circuit": passes_per_circuit},
         "s_per_circuit": round(s_per_circuit, 4),
         "roofline": {"bound": "hbm", "achieved": round(chunk_gbps, 1), "peak": PEAK_HBM_GBPS,
-                     "unit": "GB/s", "frac": round(chunk_gbps / PEAK_HBM_GBPS, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(chunk_gbps / PEAK_HBM_GBPS, 4),
+                     "traffic": traffic and round(traffic["chunk_pass"]["traffic_bytes_per_launch"]),
                      "kernel": "k_tile_pass<float,1> (LDS chunk pass)",
                      "launches": cp_n, "avg_ms": round(chunk_avg_ms, 4)},
         "permute_kernel": {"achieved": round(perm_gbps, 1), "frac": round(perm_gbps / PEAK_HBM_GBPS, 4),
+                           "traffic": traffic and round(traffic["permute"]["traffic_bytes_per_launch"]),
                            "launches": pm_n, "avg_ms": round(perm_avg_ms, 4)},
         "norm_ok": tile_result_ok,
         "cpu_baseline": cpu,
