@@ -76,6 +76,8 @@ _SIGNATURES = {
     "qsim_profile": ([_P, _I], _I),
     "qsim_profile_read": ([_P, _P, _P, _I], _I),
     "qsim_plan_describe": ([_I, _I, _P, _I, _P, _U64, _I, _I, _P, _I, _P], _I),
+    "qsim_set_jit": ([_P, _I], _I),
+    "qsim_jit_compile_check": ([_I, _I, _P, _I, _P, _U64, _I, _I], _I),
 }
 
 EXPORTED = tuple(_SIGNATURES)
@@ -321,6 +323,11 @@ class _StatevectorBase:
     def set_tile_params(self, K, cmin):
         _check(lib.qsim_set_tile_params(self._h, int(K), int(cmin)))
 
+    def set_jit(self, enable=True):
+        """Run forward chunk passes with kernels specialised per pass (hiprtc,
+        compiled on first use and cached); False: the compiled slot-program kernel."""
+        _check(lib.qsim_set_jit(self._h, 1 if enable else 0))
+
     def profile(self, enable=True):
         """Time each kernel launch with HIP events on this state's stream."""
         _check(lib.qsim_profile(self._h, 1 if enable else 0))
@@ -336,9 +343,10 @@ class _StatevectorBase:
         return {names[i]: (float(ms[i]), int(cnt[i])) for i in range(m)}
 
     def counters(self):
-        buf = (ctypes.c_int64 * 5)()
-        lib.qsim_get_counters(self._h, ctypes.addressof(buf), 5)
-        keys = ("kernel_launches", "graph_launches", "graph_captures", "chunk_passes", "permutations")
+        buf = (ctypes.c_int64 * 6)()
+        lib.qsim_get_counters(self._h, ctypes.addressof(buf), 6)
+        keys = ("kernel_launches", "graph_launches", "graph_captures", "chunk_passes", "permutations",
+                "specialized_passes")
         return dict(zip(keys, list(buf)))
 
 
@@ -415,6 +423,17 @@ def validate_ops(ops, num_qubits):
     """Host-only integer checks of the C layer on packed ops."""
     arr, arena = pack_ops(ops, num_qubits, np.complex64)
     _check(lib.qsim_validate_ops(num_qubits, ctypes.addressof(arr), len(ops), arena.size))
+
+
+def jit_compile_check(num_qubits, arr, nops, arena, K, cmin):
+    """Host-only: plan packed ops and compile every specialised pass kernel for
+    gfx950 without loading it; returns the number of passes."""
+    code = C128 if arena.dtype == np.complex128 else C64
+    npass = lib.qsim_jit_compile_check(num_qubits, code, ctypes.addressof(arr), nops, arena.ctypes.data,
+                                       arena.size, K, cmin)
+    if npass < 0:
+        raise RuntimeError(lib.qsim_last_error().decode())
+    return npass
 
 
 def plan_describe(num_qubits, arr, nops, arena, K, cmin, steps=False):

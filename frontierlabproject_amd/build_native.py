@@ -15,8 +15,12 @@ CSRC = os.path.join(PKG, "csrc")
 LIBDIR = os.path.join(PKG, "lib")
 LIB = os.path.join(LIBDIR, "libqgpusim_hip.so")
 ARCH = "gfx950"
+ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 
-SOURCES = ["qsim_kernels.hip", "qsim_capi.cpp", "qsim_plan.cpp"]
+SOURCES = ["qsim_kernels.hip", "qsim_capi.cpp", "qsim_plan.cpp", "qsim_jit.cpp"]
+# device code that the run-time compiler (hiprtc) gets as one string
+PRELUDE_HEADERS = ["qsim_dev_types.h", "qsim_device.h"]
+PRELUDE_INC = os.path.join(CSRC, "qsim_jit_prelude.inc")
 
 
 def hipcc():
@@ -26,8 +30,29 @@ def hipcc():
     raise RuntimeError("hipcc not found: the HIP toolchain is required to build the backend")
 
 
+def write_prelude():
+    """Embed the shared device headers as a raw string for run-time compiles."""
+    parts = []
+    for h in PRELUDE_HEADERS:
+        with open(os.path.join(CSRC, h)) as f:
+            for line in f:
+                if line.startswith("#pragma once") or line.startswith('#include "qsim_'):
+                    continue
+                parts.append(line)
+    text = "".join(parts)
+    delim = "QSIMPRELUDE"
+    if ")" + delim in text:
+        raise RuntimeError("prelude delimiter occurs in the device headers")
+    body = "static const char kPrelude[] = R\"%s(\n%s)%s\";\n" % (delim, text, delim)
+    old = open(PRELUDE_INC).read() if os.path.exists(PRELUDE_INC) else None
+    if old != body:
+        with open(PRELUDE_INC, "w") as f:
+            f.write(body)
+
+
 def build(verbose=False, extra_flags=(), out=None):
     out = out or LIB
+    write_prelude()
     os.makedirs(os.path.dirname(out), exist_ok=True)
     tmp = out + ".tmp%d" % os.getpid()
     cmd = [hipcc(), f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-shared",
@@ -36,6 +61,7 @@ def build(verbose=False, extra_flags=(), out=None):
            "-I", os.path.join(ROOT, "include"), "-I", CSRC, "-o", tmp]
     cmd += list(extra_flags)
     cmd += ["-x", "hip"] + [os.path.join(CSRC, s) for s in SOURCES]
+    cmd += ["-L", os.path.join(ROCM, "lib"), "-lhiprtc", "-Wl,-rpath," + os.path.join(ROCM, "lib")]
     if verbose:
         print(" ".join(cmd), flush=True)
     res = subprocess.run(cmd, capture_output=True, text=True)

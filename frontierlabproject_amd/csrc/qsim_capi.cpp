@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <list>
 #include <memory>
@@ -11,6 +12,7 @@
 
 #include "../../include/qgpusim.h"
 #include "qsim_internal.h"
+#include "qsim_jit.h"
 #include "qsim_plan.h"
 
 using namespace qsim;
@@ -121,7 +123,11 @@ struct qsim_state {
     size_t small_cap = 0;          // doubles
     double *d_probe = nullptr;     // probe partials
     size_t probe_cap = 0;          // doubles
-    int64_t counters[5] = {0, 0, 0, 0, 0};
+    // kernel launches, graph launches, graph captures, chunk passes,
+    // permutations, chunk passes run by specialised kernels
+    int64_t counters[6] = {0, 0, 0, 0, 0, 0};
+    bool jit = true;               // specialise forward chunk passes at run time
+    std::string arch;              // device gcnArchName (for run-time compiles)
     // per-kernel-class timing with events on `stream` (off by default)
     bool prof = false, capturing = false;
     struct Rec {
@@ -270,7 +276,7 @@ int run_sequential(qsim_state *st, const qsim_op *ops, int nops, const void *mat
 // Enqueue the passes of `plan` whose descriptor image lives at `dev`.
 template <typename R>
 int enqueue_passes(qsim_state *st, const Plan &plan, const DescImage &img, char *dev, void *lam,
-                   double *probe_out)
+                   double *probe_out, const std::vector<hipFunction_t> *jit = nullptr)
 {
     const PassDev *dp = reinterpret_cast<const PassDev *>(dev + img.off_pass);
     const StepDev *ds = reinterpret_cast<const StepDev *>(dev + img.off_step);
@@ -280,8 +286,17 @@ int enqueue_passes(qsim_state *st, const Plan &plan, const DescImage &img, char 
         const PassDev &hp = plan.passes[p];
         const int nblocks = 1 << (st->n - hp.K);
         ProfScope ps(st, 0);
-        HIP_TRY(launch_tile_pass<R>(static_cast<cx<R> *>(st->psi), static_cast<cx<R> *>(lam), st->n, hp, dp + p, ds,
-                                    dm, st->d_probe, st->stream));
+        if (!lam && jit && p < jit->size() && (*jit)[p]) {
+            void *psi = st->psi;
+            const void *mats = dm;
+            void *args[] = {&psi, &mats};
+            HIP_TRY(hipModuleLaunchKernel((*jit)[p], (unsigned)nblocks, 1, 1, kTPB, 1, 1, 0, st->stream, args,
+                                          nullptr));
+            st->counters[5]++;
+        } else {
+            HIP_TRY(launch_tile_pass<R>(static_cast<cx<R> *>(st->psi), static_cast<cx<R> *>(lam), st->n, hp, dp + p,
+                                        ds, dm, st->d_probe, st->stream));
+        }
         st->counters[0]++;
         st->counters[3]++;
         if (lam && hp.nprobe > 0) {
@@ -302,17 +317,34 @@ int plan_for(qsim_state *st, const qsim_op *ops, int nops, const void *mats, int
     return 0;
 }
 
+// Specialised kernels for the plan's passes (empty when switched off).
+int jit_for(qsim_state *st, const Plan &plan, bool dbl, std::vector<hipFunction_t> &fns)
+{
+    fns.clear();
+    if (!st->jit) return 0;
+    if (st->arch.empty()) {
+        hipDeviceProp_t prop;
+        HIP_TRY(hipGetDeviceProperties(&prop, st->device));
+        st->arch = prop.gcnArchName;
+    }
+    std::string err;
+    if (!jit_functions(plan, dbl, st->device, st->arch, fns, err)) return fail("run-time kernel compile: " + err);
+    return 0;
+}
+
 template <typename R>
 int run_chunked(qsim_state *st, const qsim_op *ops, int nops, const void *mats, uint64_t mat_count)
 {
     (void)mat_count;
     Plan plan;
     if (plan_for(st, ops, nops, mats, 1, plan)) return -1;
+    std::vector<hipFunction_t> fns;
+    if (jit_for(st, plan, sizeof(R) == 8, fns)) return -1;
     DescImage img;
     img.build<R>(plan);
     void *dev = nullptr;
     if (st->up.upload(img.bytes.data(), img.bytes.size(), st->stream, &dev)) return -1;
-    return enqueue_passes<R>(st, plan, img, static_cast<char *>(dev), nullptr, nullptr);
+    return enqueue_passes<R>(st, plan, img, static_cast<char *>(dev), nullptr, nullptr, &fns);
 }
 
 template <typename R>
@@ -327,6 +359,7 @@ int run_graphed(qsim_state *st, const qsim_op *ops, int nops, const void *mats, 
     put(&st->psi, sizeof(void *));
     put(&st->K, sizeof(int));
     put(&st->cmin, sizeof(int));
+    put(&st->jit, sizeof(bool));
     put(ops, sizeof(qsim_op) * nops);
     put(mats, sizeof(cx<R>) * mat_count);
     for (auto it = st->graphs.begin(); it != st->graphs.end(); ++it) {
@@ -340,6 +373,8 @@ int run_graphed(qsim_state *st, const qsim_op *ops, int nops, const void *mats, 
     }
     Plan plan;
     if (plan_for(st, ops, nops, mats, 1, plan)) return -1;
+    std::vector<hipFunction_t> fns;
+    if (jit_for(st, plan, sizeof(R) == 8, fns)) return -1;
     DescImage img;
     img.build<R>(plan);
     GraphEntry e;
@@ -349,7 +384,7 @@ int run_graphed(qsim_state *st, const qsim_op *ops, int nops, const void *mats, 
     HIP_TRY(hipStreamSynchronize(st->stream));
     HIP_TRY(hipStreamBeginCapture(st->stream, hipStreamCaptureModeThreadLocal));
     st->capturing = true;
-    int rc = enqueue_passes<R>(st, plan, img, static_cast<char *>(e.desc), nullptr, nullptr);
+    int rc = enqueue_passes<R>(st, plan, img, static_cast<char *>(e.desc), nullptr, nullptr, &fns);
     st->capturing = false;
     hipGraph_t g = nullptr;
     hipError_t ec = hipStreamEndCapture(st->stream, &g);
@@ -586,6 +621,7 @@ int qsim_create(int num_qubits, int dtype, qsim_state **out)
     st->dim = 1ull << num_qubits;
     st->K = dtype == QSIM_C64 ? 12 : 11;
     st->cmin = dtype == QSIM_C64 ? 4 : 3;
+    if (const char *e = std::getenv("QSIM_JIT")) st->jit = std::atoi(e) != 0;
     HIP_TRY(hipGetDevice(&st->device));
     static bool prepared = false;
     if (!prepared) {
@@ -893,6 +929,27 @@ int qsim_set_tile_params(qsim_state *st, int K, int cmin)
     return 0;
 }
 
+int qsim_set_jit(qsim_state *st, int enable)
+{
+    if (!st) return fail("null state");
+    st->jit = enable != 0;
+    return 0;
+}
+
+int qsim_jit_compile_check(int num_qubits, int dtype, const qsim_op *ops, int nops, const void *mats,
+                           uint64_t mat_count, int K, int cmin)
+{
+    if (dtype != QSIM_C64 && dtype != QSIM_C128) return fail("unknown dtype");
+    if (check_ops(num_qubits, ops, nops, mat_count, false)) return -1;
+    std::vector<PlanOp> pops = to_plan_ops(ops, nops);
+    Plan plan;
+    std::string err;
+    if (!plan_passes(num_qubits, pops.data(), nops, mats, (int)amp_bytes(dtype), K, cmin, 1, plan, err))
+        return fail(err);
+    if (!jit_compile_only(plan, dtype == QSIM_C128, "gfx950", err)) return fail("run-time kernel compile: " + err);
+    return (int)plan.passes.size();
+}
+
 int qsim_profile(qsim_state *st, int enable)
 {
     if (!st) return fail("null state");
@@ -926,7 +983,7 @@ int qsim_profile_read(qsim_state *st, double *ms_out, int64_t *count_out, int nc
 int qsim_get_counters(const qsim_state *st, int64_t *out, int n)
 {
     if (!st || !out) return fail("null argument");
-    const int m = std::min(n, 5);
+    const int m = std::min(n, 6);
     for (int i = 0; i < m; ++i) out[i] = st->counters[i];
     return m;
 }

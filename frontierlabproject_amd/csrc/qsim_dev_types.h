@@ -1,0 +1,80 @@
+// Types and constants shared by the host planner, the compiled kernels and the
+// chunk-pass kernels compiled at run time.  Uses no standard header when built
+// by hiprtc (__HIPCC_RTC__), which supplies the fixed-width integer types.
+#pragma once
+#ifdef __HIPCC_RTC__
+typedef __hip_internal::uint64_t uint64_t;
+typedef __hip_internal::uint32_t uint32_t;
+typedef __hip_internal::int32_t int32_t;
+#else
+#include <cstdint>
+#endif
+
+namespace qsim {
+
+template <typename R> struct cx { R re, im; };
+
+constexpr int kTPB = 256;              // threads per block for every kernel
+constexpr int kWaves = kTPB / 64;
+
+constexpr int kMaxK = 14;      // chunk bits
+constexpr int kG = 4;          // register-step bits
+constexpr int kMaxComp = 50;   // bits outside a chunk
+#ifndef QSIM_ROUNDS
+#define QSIM_ROUNDS 4
+#endif
+constexpr int kRounds = QSIM_ROUNDS;  // maximum slot rounds per step
+constexpr int kSlotsPerRound = 10;
+constexpr int kSlots = kRounds * kSlotsPerRound;
+constexpr int k1qSlots = kRounds * 4;
+
+// Slot flags.  Each gate is classified exactly (entries compared with 0, 1, i)
+// so the kernel runs the cheapest arithmetic for it.  1q slots (x: bit 0
+// amplitude, y: bit 1 amplitude of each pair):
+enum : int32_t {
+    S1_SKIP = 0,
+    S1_GEN = 1,        // [[u0,u1],[u2,u3]]                     4 cx
+    S1_DIAG = 2,       // diag(d0, d1)                           2 cx
+    S1_PROBE = 3,      // gradient probe with G                  8 cx (G, then identity)
+    S1_PROBE_GEN = 4,  // probe with G, then apply U             8 cx (G, U)
+    S1_PHASE = 5,      // diag(1, d1)                            1 cx
+    S1_REAL = 6,       // real matrix                            2 cx (r0,r1),(r2,r3)
+    S1_RI = 7,         // [[a, ib],[ic, d]], a b c d real        2 cx (a,b),(c,d)
+    S1_X = 8,          // swap x, y
+    S1_Y = 9,          // x' = -i y, y' = i x
+    S1_BFLY = 10,      // x' = x + y, y' = x - y (H without its 1/sqrt2)
+    S1_Z = 11,         // y' = -y
+    S1_PI = 12,        // y' = i y   (S)
+    S1_MI = 13,        // y' = -i y  (S^dag)
+};
+// pair slots (matrix index b_first + 2 b_second, first < second register bit);
+// other pair ops make the step dense:
+enum : int32_t {
+    S2_SKIP = 0,
+    S2_CPHASE = 5,       // diag(1, 1, 1, d3)                     1 cx
+    S2_CX_FIRST = 6,     // flip second where first = 1
+    S2_CX_SECOND = 7,    // flip first where second = 1
+    S2_SWAP = 8,         // exchange index 1 and 2
+};
+
+// Device-side records (plain int32 so one upload carries them).
+struct PassDev {
+    int32_t K, cfast, ncomp, nsteps;
+    int32_t step_off, nprobe, probe_off, pad;
+    int32_t pos[kMaxK];          // chunk bit k -> state bit position (ascending)
+    int32_t comp[kMaxComp];      // block-index bit m -> state bit position
+};
+
+struct StepDev {
+    int32_t bits[kG];            // register bit t -> chunk-local bit
+    int32_t sbits[kG];           // the same bits, ascending
+    int32_t nrounds;             // slot rounds in use
+    int32_t dense;               // arena offset of a 16x16 matrix (dense step) or -1
+    int32_t scale;               // arena offset of a factor applied after the step, or -1
+    int32_t pad;
+    int32_t kind[kSlots];        // slot flags
+    int32_t mat[kSlots];         // complex-element offsets into the plan's arena
+    int32_t probe[k1qSlots];     // probe slot within the pass, per 1q slot
+};
+
+}  // namespace qsim

@@ -5,16 +5,13 @@
 #include <cstdint>
 
 #include "qsim_plan.h"
+#include "qsim_device.h"
 
 namespace qsim {
-
-template <typename R> struct cx { R re, im; };
 
 template <typename R> struct Mat1 { cx<R> u[4]; };
 template <typename R> struct Mat2 { cx<R> u[16]; };
 
-constexpr int kTPB = 256;              // threads per block for every kernel
-constexpr int kWaves = kTPB / 64;
 constexpr int kMaxProbes = 256;        // probe slots per chunk pass
 constexpr int kPermMaxS = 16;
 

@@ -1,0 +1,278 @@
+// Run-time specialisation of chunk passes (see qsim_jit.h).
+#include "qsim_jit.h"
+
+#include <hip/hiprtc.h>
+
+#include <algorithm>
+#include <atomic>
+#include <cstdio>
+#include <cstdlib>
+#include <map>
+#include <mutex>
+#include <sstream>
+#include <thread>
+#include <utility>
+
+namespace qsim {
+
+namespace {
+
+// qsim_dev_types.h + qsim_device.h as one string (generated by the build).
+#include "qsim_jit_prelude.inc"
+
+std::mutex g_mu;
+std::map<std::pair<int, std::string>, hipFunction_t> g_fns;  // (device, source) -> function
+std::vector<hipModule_t> g_modules;                           // never unloaded
+std::atomic<long> g_compiled{0};
+
+int swz_host(int e) { return e ^ (((e >> 6) & 31) << 1); }
+
+uint64_t fnv1a(const std::string &s)
+{
+    uint64_t h = 1469598103934665603ull;
+    for (unsigned char c : s) {
+        h ^= c;
+        h *= 1099511628211ull;
+    }
+    return h;
+}
+
+bool pass_has_probes(const Plan &plan, size_t p) { return plan.passes[p].nprobe > 0; }
+
+// Pair slots of a round: register bits (first, second).
+constexpr int kPairF[6] = {0, 2, 0, 1, 0, 1};
+constexpr int kPairS[6] = {1, 3, 2, 3, 3, 2};
+
+void emit_step(std::ostringstream &o, const StepDev &sd, int E, const char *T)
+{
+    const int NG = E >> kG;
+    int moff[16];
+    for (int m = 0; m < 16; ++m)
+        moff[m] = swz_host(((m & 1) << sd.bits[0]) | (((m >> 1) & 1) << sd.bits[1]) |
+                           (((m >> 2) & 1) << sd.bits[2]) | (((m >> 3) & 1) << sd.bits[3]));
+    const bool loop = NG > kTPB, guard = NG < kTPB;
+    o << "  {\n";
+    if (loop) o << "  for (int gb = 0; gb < " << NG << "; gb += kTPB) {\n";
+    o << "    const int gi = " << (loop ? "gb + " : "") << "(int)threadIdx.x;\n";
+    if (guard)
+        o << "    const bool act = gi < " << NG << ";\n    const uint32_t g32 = (uint32_t)(act ? gi : " << NG - 1
+          << ");\n";
+    else
+        o << "    const uint32_t g32 = (uint32_t)gi;\n";
+    o << "    const int sb = swz((int)ins0(ins0(ins0(ins0(g32, " << sd.sbits[0] << "), " << sd.sbits[1] << "), "
+      << sd.sbits[2] << "), " << sd.sbits[3] << "));\n";
+    o << "    cx<" << T << "> a[16];\n";
+    for (int m = 0; m < 16; ++m) o << "    a[" << m << "] = c0[sb ^ " << moff[m] << "];\n";
+    const char *store_guard = guard ? "if (act) " : "";
+    if (sd.dense >= 0) {
+        if (sd.scale >= 0) o << "    const cx<" << T << "> f = mats[" << sd.scale << "];\n";
+        for (int r = 0; r < 16; ++r) {
+            const int M = sd.dense + 16 * r;
+            o << "    { cx<" << T << "> acc = cmul(mats[" << M << "], a[0]);\n";
+            o << "      #pragma unroll\n      for (int c = 1; c < 16; ++c) acc = cmad(mats[" << M
+              << " + c], a[c], acc);\n";
+            if (sd.scale >= 0) o << "      acc = cmul(acc, f);\n";
+            o << "      " << store_guard << "c0[sb ^ " << moff[r] << "] = acc; }\n";
+        }
+    } else {
+        for (int r = 0; r < sd.nrounds; ++r) {
+            const int b = r * kSlotsPerRound;
+            for (int j = 0; j < 4; ++j) {
+                const int k = sd.kind[b + j];
+                if (k == S1_SKIP) continue;
+                o << "    apply1<" << T << ", " << j << ">(a, " << k << ", mats + " << sd.mat[b + j] << ");\n";
+            }
+            for (int j = 0; j < 6; ++j) {
+                const int k = sd.kind[b + 4 + j];
+                if (k == S2_SKIP) continue;
+                o << "    apply2<" << T << ", " << kPairF[j] << ", " << kPairS[j] << ">(a, " << k << ", mats + "
+                  << sd.mat[b + 4 + j] << ");\n";
+            }
+        }
+        if (sd.scale >= 0) {
+            o << "    { const cx<" << T << "> f = mats[" << sd.scale << "];\n";
+            o << "      #pragma unroll\n      for (int m = 0; m < 16; ++m) a[m] = cmul(a[m], f); }\n";
+        }
+        for (int m = 0; m < 16; ++m) o << "    " << store_guard << "c0[sb ^ " << moff[m] << "] = a[" << m << "];\n";
+    }
+    if (loop) o << "  }\n";
+    o << "  }\n  __syncthreads();\n";
+}
+
+std::string kernel_name(const std::string &body) { return "qsim_jit_" + std::to_string(fnv1a(body)); }
+
+std::string arch_flag(const std::string &arch) { return "--offload-arch=" + (arch.empty() ? "gfx950" : arch); }
+
+// Compile one translation unit; returns the code object or sets err.
+bool compile_unit(const std::string &src, const std::string &arch, std::vector<char> &code, std::string &err)
+{
+    hiprtcProgram prog = nullptr;
+    hiprtcResult r = hiprtcCreateProgram(&prog, src.c_str(), "qsim_jit.hip", 0, nullptr, nullptr);
+    if (r != HIPRTC_SUCCESS) {
+        err = std::string("hiprtcCreateProgram: ") + hiprtcGetErrorString(r);
+        return false;
+    }
+    const std::string af = arch_flag(arch);
+    const char *opts[] = {af.c_str(), "-O3", "-std=c++17", "-ffp-contract=fast",
+                          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"};
+    r = hiprtcCompileProgram(prog, (int)(sizeof(opts) / sizeof(opts[0])), opts);
+    if (r != HIPRTC_SUCCESS) {
+        size_t n = 0;
+        hiprtcGetProgramLogSize(prog, &n);
+        std::string log(n, '\0');
+        if (n) hiprtcGetProgramLog(prog, &log[0]);
+        err = std::string("hiprtcCompileProgram: ") + hiprtcGetErrorString(r) + "\n" + log.substr(0, 4000);
+        hiprtcDestroyProgram(&prog);
+        return false;
+    }
+    size_t n = 0;
+    hiprtcGetCodeSize(prog, &n);
+    code.resize(n);
+    hiprtcGetCode(prog, code.data());
+    hiprtcDestroyProgram(&prog);
+    return true;
+}
+
+// Compile `bodies` as up to `threads` units in parallel.
+bool compile_all(const std::vector<std::string> &bodies, const std::string &arch,
+                 std::vector<std::vector<char>> &codes, std::vector<std::vector<int>> &members, std::string &err)
+{
+    int hw = (int)std::thread::hardware_concurrency();
+    if (const char *e = std::getenv("QSIM_JIT_THREADS")) hw = std::atoi(e);
+    const int nu = std::max(1, std::min({(int)bodies.size(), hw > 0 ? hw : 4, 16}));
+    members.assign(nu, {});
+    for (size_t i = 0; i < bodies.size(); ++i) members[i % nu].push_back((int)i);
+    std::vector<std::string> srcs(nu);
+    for (int u = 0; u < nu; ++u) {
+        std::string s = kPrelude;
+        s += "\nnamespace qsim {\n";
+        for (int i : members[u]) s += bodies[i];
+        s += "}  // namespace qsim\n";
+        srcs[u] = std::move(s);
+    }
+    if (const char *dir = std::getenv("QSIM_JIT_DUMP")) {  // sources for inspection
+        for (int u = 0; u < nu; ++u) {
+            const std::string path = std::string(dir) + "/qsim_jit_" + std::to_string(fnv1a(srcs[u])) + ".hip";
+            if (FILE *f = std::fopen(path.c_str(), "w")) {
+                std::fwrite(srcs[u].data(), 1, srcs[u].size(), f);
+                std::fclose(f);
+            }
+        }
+    }
+    codes.assign(nu, {});
+    std::vector<std::string> errs(nu);
+    std::vector<char> ok(nu, 0);
+    std::vector<std::thread> th;
+    for (int u = 0; u < nu; ++u)
+        th.emplace_back([&, u] { ok[u] = compile_unit(srcs[u], arch, codes[u], errs[u]); });
+    for (auto &t : th) t.join();
+    for (int u = 0; u < nu; ++u)
+        if (!ok[u]) {
+            err = errs[u];
+            return false;
+        }
+    g_compiled += (long)bodies.size();
+    return true;
+}
+
+}  // namespace
+
+std::string jit_pass_source(const Plan &plan, size_t p, bool dbl, const std::string &name)
+{
+    const PassDev &pd = plan.passes[p];
+    const char *T = dbl ? "double" : "float";
+    const int K = pd.K, CF = pd.cfast, E = 1 << K, NROW = 1 << (K - CF);
+    std::ostringstream o;
+    o << "extern \"C\" __global__ __launch_bounds__(kTPB)"
+      << (dbl ? "" : " __attribute__((amdgpu_waves_per_eu(4)))") << "\nvoid " << name << "(cx<" << T
+      << "> *__restrict__ psi, const cx<" << T << "> *__restrict__ mats)\n{\n";
+    o << "  __shared__ __align__(16) cx<" << T << "> c0[" << E << "];\n";
+    o << "  __shared__ uint64_t rowoff[" << NROW << "];\n";
+    o << "  for (int t = threadIdx.x; t < " << NROW << "; t += kTPB) {\n    uint64_t r = 0;\n";
+    for (int k = CF; k < K; ++k)
+        o << "    r |= (uint64_t)((t >> " << k - CF << ") & 1) << " << pd.pos[k] << ";\n";
+    o << "    rowoff[t] = r;\n  }\n";
+    o << "  const uint64_t c = blockIdx.x;\n  uint64_t base = 0;\n";
+    for (int m = 0; m < pd.ncomp;) {  // runs of consecutive positions move as one field
+        int len = 1;
+        while (m + len < pd.ncomp && pd.comp[m + len] == pd.comp[m] + len) ++len;
+        o << "  base |= ((c >> " << m << ") & " << ((1ull << len) - 1) << "ull) << " << pd.comp[m] << ";\n";
+        m += len;
+    }
+    o << "  __syncthreads();\n";
+    o << "  chunk_copy<" << T << ", true>(psi, c0, base, rowoff, " << E << ", " << CF << ");\n";
+    o << "  __syncthreads();\n";
+    for (int s = 0; s < pd.nsteps; ++s) emit_step(o, plan.steps[pd.step_off + s], E, T);
+    o << "  chunk_copy<" << T << ", false>(psi, c0, base, rowoff, " << E << ", " << CF << ");\n}\n\n";
+    return o.str();
+}
+
+bool jit_functions(const Plan &plan, bool dbl, int device, const std::string &arch,
+                   std::vector<hipFunction_t> &out, std::string &err)
+{
+    const size_t np = plan.passes.size();
+    out.assign(np, nullptr);
+    std::vector<std::string> keys(np);
+    std::vector<std::string> missing_bodies;
+    std::vector<std::string> missing_keys;
+    {
+        std::lock_guard<std::mutex> lk(g_mu);
+        std::map<std::string, int> seen;
+        for (size_t p = 0; p < np; ++p) {
+            if (pass_has_probes(plan, p)) continue;
+            keys[p] = jit_pass_source(plan, p, dbl, "@");
+            auto it = g_fns.find({device, keys[p]});
+            if (it != g_fns.end()) {
+                out[p] = it->second;
+            } else if (!seen.count(keys[p])) {
+                seen[keys[p]] = 1;
+                missing_keys.push_back(keys[p]);
+                missing_bodies.push_back(jit_pass_source(plan, p, dbl, kernel_name(keys[p])));
+            }
+        }
+    }
+    if (!missing_bodies.empty()) {
+        std::vector<std::vector<char>> codes;
+        std::vector<std::vector<int>> members;
+        if (!compile_all(missing_bodies, arch, codes, members, err)) return false;
+        std::lock_guard<std::mutex> lk(g_mu);
+        for (size_t u = 0; u < codes.size(); ++u) {
+            hipModule_t mod = nullptr;
+            hipError_t e = hipModuleLoadData(&mod, codes[u].data());
+            if (e != hipSuccess) {
+                err = std::string("hipModuleLoadData: ") + hipGetErrorString(e);
+                return false;
+            }
+            g_modules.push_back(mod);
+            for (int i : members[u]) {
+                hipFunction_t f = nullptr;
+                e = hipModuleGetFunction(&f, mod, kernel_name(missing_keys[i]).c_str());
+                if (e != hipSuccess) {
+                    err = std::string("hipModuleGetFunction: ") + hipGetErrorString(e);
+                    return false;
+                }
+                g_fns[{device, missing_keys[i]}] = f;
+            }
+        }
+        for (size_t p = 0; p < np; ++p)
+            if (!keys[p].empty() && !out[p]) out[p] = g_fns[{device, keys[p]}];
+    }
+    return true;
+}
+
+bool jit_compile_only(const Plan &plan, bool dbl, const std::string &arch, std::string &err)
+{
+    std::vector<std::string> bodies;
+    for (size_t p = 0; p < plan.passes.size(); ++p) {
+        if (pass_has_probes(plan, p)) continue;
+        bodies.push_back(jit_pass_source(plan, p, dbl, kernel_name(jit_pass_source(plan, p, dbl, "@"))));
+    }
+    if (bodies.empty()) return true;
+    std::vector<std::vector<char>> codes;
+    std::vector<std::vector<int>> members;
+    return compile_all(bodies, arch, codes, members, err);
+}
+
+long jit_compiled_count() { return g_compiled.load(); }
+
+}  // namespace qsim

@@ -1,0 +1,37 @@
+// Chunk-pass kernels specialised per pass and compiled at run time (hiprtc).
+//
+// The compiled kernel k_tile_pass reads each step's slot program from device
+// memory and branches on every slot's gate kind, so each slot ends in a join
+// where the 16 amplitudes are copied back into fixed registers.  A specialised
+// kernel has the pass's register bits, LDS offsets and gate kinds as
+// constants: the slots become straight-line code, and register exchanges (X,
+// CX, SWAP) become renames.  Arithmetic and order are those of k_tile_pass;
+// both include qsim_device.h.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "qsim_plan.h"
+
+namespace qsim {
+
+// One function per pass of `plan` (nullptr for passes that stay on
+// k_tile_pass: those with gradient probes).  Kernels are cached per device
+// and source for the life of the process; missing ones are compiled in
+// parallel.  `arch` is the device's gcnArchName.  Returns false with `err`.
+bool jit_functions(const Plan &plan, bool dbl, int device, const std::string &arch,
+                   std::vector<hipFunction_t> &out, std::string &err);
+
+// Source of the specialised kernel for pass `p` (kernel name `name`).
+std::string jit_pass_source(const Plan &plan, size_t p, bool dbl, const std::string &name);
+
+// Compile every pass of `plan` for `arch` without loading it (no device
+// needed).  Returns false with the compiler log in `err`.
+bool jit_compile_only(const Plan &plan, bool dbl, const std::string &arch, std::string &err);
+
+// Kernels compiled so far in this process (all devices).
+long jit_compiled_count();
+
+}  // namespace qsim

@@ -150,6 +150,21 @@ int qsim_adjoint(qsim_state *st, const qsim_pauli *terms, int nterms, const qsim
 /* Chunk-pass geometry: K chunk bits, cmin always-present low bits. */
 int qsim_set_tile_params(qsim_state *st, int K, int cmin);
 
+/* No reference counterpart (the reference has one fixed kernel per gate).
+ * 1 (default; environment QSIM_JIT=0 switches it off at qsim_create):
+ * forward chunk passes run kernels specialised per pass, with the pass's
+ * register bits, LDS offsets and gate kinds compiled in by hiprtc on first
+ * use and cached per device for the life of the process.  0: every pass runs
+ * the compiled slot-program kernel.  Both apply the same arithmetic in the
+ * same order. */
+int qsim_set_jit(qsim_state *st, int enable);
+
+/* Host only, no reference counterpart: plan `ops` as qsim_apply_ops would and
+ * compile (without loading) every specialised pass kernel for gfx950.
+ * Returns the number of passes, or -1 with the compiler log. */
+int qsim_jit_compile_check(int num_qubits, int dtype, const qsim_op *ops, int nops,
+                           const void *mats, uint64_t mat_count, int K, int cmin);
+
 /* Per-kernel-class timing with HIP events recorded on the state's stream around
  * each launch (not inside captured graphs).  Classes: 0 chunk pass,
  * 1 permutation, 2 gate stream, 3 Pauli chunk pass. */
@@ -160,7 +175,8 @@ int qsim_profile(qsim_state *st, int enable);
 int qsim_profile_read(qsim_state *st, double *ms_out, int64_t *count_out, int nclasses);
 
 /* Counters: [0] kernel launches, [1] graph launches, [2] graph captures,
- * [3] chunk passes, [4] permutations.  Returns the number written. */
+ * [3] chunk passes, [4] permutations, [5] chunk passes run by specialised
+ * kernels.  Returns the number written. */
 int qsim_get_counters(const qsim_state *st, int64_t *out, int n);
 
 /* Host only: plan `ops` into chunk passes without a device.  Writes, per pass,

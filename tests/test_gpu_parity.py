@@ -203,6 +203,52 @@ def test_deep_factored_gates(hip, dtype):
         _check(ref, sv.to_numpy(), dtype)
 
 
+def _mixed_ops(n, seed):
+    """Random-circuit layers plus CZ/CP/SWAP/general 4x4 (dense steps) and
+    long H/T/RZ runs (factored scalars)."""
+    rng = np.random.default_rng(seed)
+    ops = _tuples(C.random_circuit(n, 6, seed))
+    for _ in range(30):
+        a, b = (int(x) for x in rng.choice(n, 2, replace=False))
+        U = [G.cx(), G.cz(), G.cp(rng.uniform(0, 6)), G.swap(),
+             np.linalg.qr(rng.standard_normal((4, 4)) + 1j * rng.standard_normal((4, 4)))[0]][int(rng.integers(0, 5))]
+        ops.append(("2q", a, b, U))
+        ops.append(("1q", a, [G.h(), G.t(), G.rz(rng.uniform(0, 6)), G.y()][int(rng.integers(0, 4))]))
+    return ops
+
+
+@pytest.mark.parametrize("dtype,K,cmin", [(np.complex64, 6, 2), (np.complex64, 9, 3), (np.complex64, 12, 4),
+                                          (np.complex64, 13, 4), (np.complex128, 8, 3), (np.complex128, 11, 3),
+                                          (np.complex128, 12, 3)])
+def test_specialised_passes_match_slot_kernel(hip, dtype, K, cmin):
+    """Chunk passes compiled per pass at run time against the FP64 oracle and
+    against the slot-program kernel on the same plan."""
+    n = 14
+    ops = _mixed_ops(n, 7 + K)
+    psi0 = C.random_state(n, 5)
+    ref = psi0
+    for o in ops:
+        ref = dense.apply(ref, o[-1], list(o[1:-1]), n)
+    out = {}
+    for jit in (False, True):
+        for mode in ("apply_tile_cooperative", "apply_tile_graphed"):
+            sv = _sv(hip, n, dtype)
+            sv.set_tile_params(K, cmin)
+            sv.set_jit(jit)
+            sv.from_numpy(psi0)
+            getattr(sv, mode)(ops)
+            got = sv.to_numpy()
+            _check(ref, got, dtype)
+            c = sv.counters()
+            assert c["chunk_passes"] > 0
+            assert (c["specialized_passes"] == c["chunk_passes"]) if jit else c["specialized_passes"] == 0
+            out[jit, mode] = got
+    # same arithmetic in the same order: agreement far inside the oracle tolerance
+    scale = 1e-6 if dtype == np.complex64 else 1e-14
+    for mode in ("apply_tile_cooperative", "apply_tile_graphed"):
+        assert np.max(np.abs(out[True, mode] - out[False, mode])) < scale
+
+
 def test_graph_replay_hits(hip):
     n = 12
     ops = _tuples(C.random_circuit(n, 6, 5))
