@@ -63,6 +63,43 @@ class _OpBuffer:
         return np.ascontiguousarray(np.concatenate(self.mats), dtype=self.dtype)
 
 
+def _norm_spec(spec):
+    return [tuple(s) + ((),) if len(s) == 3 else tuple(s) for s in spec]
+
+
+def forward_ops(spec, theta, dtype):
+    """Packed op list of the circuit at parameters theta."""
+    spec = _norm_spec(spec)
+    buf = _OpBuffer(len(spec), dtype)
+    for name, qs, k, fixed in spec:
+        m = _matrix(name, None if k is None else float(theta[k]), fixed)
+        if len(qs) == 1:
+            buf.add(hip.OP_1Q, qs[0], -1, m)
+        else:
+            buf.add(hip.OP_2Q, qs[0], qs[1], m)
+    return buf
+
+
+def reverse_ops(spec, theta, dtype):
+    """Packed reverse sweep: inverse gates in reverse order, each parameterised
+    gate preceded by a probe of its generator in the parameter's slot."""
+    spec = _norm_spec(spec)
+    nprobe = sum(1 for s in spec if s[2] is not None)
+    buf = _OpBuffer(len(spec) + nprobe, dtype)
+    slot = nprobe
+    for name, qs, k, fixed in reversed(spec):
+        m = _matrix(name, None if k is None else float(theta[k]), fixed)
+        if k is not None:
+            slot -= 1
+            buf.add(hip.OP_PROBE, qs[0], slot, GENERATORS[name])
+        mdag = np.conj(m).T
+        if len(qs) == 1:
+            buf.add(hip.OP_1Q, qs[0], -1, mdag)
+        else:
+            buf.add(hip.OP_2Q, qs[0], qs[1], mdag)
+    return buf, nprobe
+
+
 class AdjointSimulator:
     """Expectation and adjoint gradient of a parameterised circuit.
 
@@ -85,30 +122,10 @@ class AdjointSimulator:
         self._probe_param = [k for _, _, k, _ in self.spec if k is not None]
 
     def _forward_ops(self, theta):
-        buf = _OpBuffer(len(self.spec), self.dtype)
-        for name, qs, k, fixed in self.spec:
-            m = _matrix(name, None if k is None else float(theta[k]), fixed)
-            if len(qs) == 1:
-                buf.add(hip.OP_1Q, qs[0], -1, m)
-            else:
-                buf.add(hip.OP_2Q, qs[0], qs[1], m)
-        return buf
+        return forward_ops(self.spec, theta, self.dtype)
 
     def _reverse_ops(self, theta):
-        nprobe = len(self._probe_param)
-        buf = _OpBuffer(len(self.spec) + nprobe, self.dtype)
-        slot = nprobe
-        for name, qs, k, fixed in reversed(self.spec):
-            m = _matrix(name, None if k is None else float(theta[k]), fixed)
-            if k is not None:
-                slot -= 1
-                buf.add(hip.OP_PROBE, qs[0], slot, GENERATORS[name])
-            mdag = np.conj(m).T
-            if len(qs) == 1:
-                buf.add(hip.OP_1Q, qs[0], -1, mdag)
-            else:
-                buf.add(hip.OP_2Q, qs[0], qs[1], mdag)
-        return buf, nprobe
+        return reverse_ops(self.spec, theta, self.dtype)
 
     def prepare(self, theta, psi0=None):
         """Run the circuit: the state holds psi(theta) afterwards."""

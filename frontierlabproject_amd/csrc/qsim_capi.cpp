@@ -286,12 +286,15 @@ int enqueue_passes(qsim_state *st, const Plan &plan, const DescImage &img, char 
         const PassDev &hp = plan.passes[p];
         const int nblocks = 1 << (st->n - hp.K);
         ProfScope ps(st, 0);
-        if (!lam && jit && p < jit->size() && (*jit)[p]) {
+        if (jit && p < jit->size() && (*jit)[p]) {
             void *psi = st->psi;
+            void *lamp = lam;
+            double *part = st->d_probe;
             const void *mats = dm;
-            void *args[] = {&psi, &mats};
-            HIP_TRY(hipModuleLaunchKernel((*jit)[p], (unsigned)nblocks, 1, 1, kTPB, 1, 1, 0, st->stream, args,
-                                          nullptr));
+            void *args1[] = {&psi, &mats};
+            void *args2[] = {&psi, &lamp, &part, &mats};
+            HIP_TRY(hipModuleLaunchKernel((*jit)[p], (unsigned)nblocks, 1, 1, kTPB, 1, 1, 0, st->stream,
+                                          lam ? args2 : args1, nullptr));
             st->counters[5]++;
         } else {
             HIP_TRY(launch_tile_pass<R>(static_cast<cx<R> *>(st->psi), static_cast<cx<R> *>(lam), st->n, hp, dp + p,
@@ -318,7 +321,7 @@ int plan_for(qsim_state *st, const qsim_op *ops, int nops, const void *mats, int
 }
 
 // Specialised kernels for the plan's passes (empty when switched off).
-int jit_for(qsim_state *st, const Plan &plan, bool dbl, std::vector<hipFunction_t> &fns)
+int jit_for(qsim_state *st, const Plan &plan, bool dbl, int nv, std::vector<hipFunction_t> &fns)
 {
     fns.clear();
     if (!st->jit) return 0;
@@ -328,7 +331,8 @@ int jit_for(qsim_state *st, const Plan &plan, bool dbl, std::vector<hipFunction_
         st->arch = prop.gcnArchName;
     }
     std::string err;
-    if (!jit_functions(plan, dbl, st->device, st->arch, fns, err)) return fail("run-time kernel compile: " + err);
+    if (!jit_functions(plan, dbl, nv, st->device, st->arch, fns, err))
+        return fail("run-time kernel compile: " + err);
     return 0;
 }
 
@@ -339,7 +343,7 @@ int run_chunked(qsim_state *st, const qsim_op *ops, int nops, const void *mats, 
     Plan plan;
     if (plan_for(st, ops, nops, mats, 1, plan)) return -1;
     std::vector<hipFunction_t> fns;
-    if (jit_for(st, plan, sizeof(R) == 8, fns)) return -1;
+    if (jit_for(st, plan, sizeof(R) == 8, 1, fns)) return -1;
     DescImage img;
     img.build<R>(plan);
     void *dev = nullptr;
@@ -374,7 +378,7 @@ int run_graphed(qsim_state *st, const qsim_op *ops, int nops, const void *mats, 
     Plan plan;
     if (plan_for(st, ops, nops, mats, 1, plan)) return -1;
     std::vector<hipFunction_t> fns;
-    if (jit_for(st, plan, sizeof(R) == 8, fns)) return -1;
+    if (jit_for(st, plan, sizeof(R) == 8, 1, fns)) return -1;
     DescImage img;
     img.build<R>(plan);
     GraphEntry e;
@@ -587,11 +591,13 @@ int adjoint_t(qsim_state *st, const qsim_pauli *terms, int nterms, const qsim_op
     HIP_TRY(hipMalloc(reinterpret_cast<void **>(&d_out), sizeof(double) * std::max(1, nprobes)));
     std::unique_ptr<void, void (*)(void *)> guard(d_out, [](void *p) { hipFree(p); });
     HIP_TRY(hipMemsetAsync(d_out, 0, sizeof(double) * std::max(1, nprobes), st->stream));
+    std::vector<hipFunction_t> fns;
+    if (jit_for(st, plan, sizeof(R) == 8, 2, fns)) return -1;
     DescImage img;
     img.build<R>(plan);
     void *dev = nullptr;
     if (st->up.upload(img.bytes.data(), img.bytes.size(), st->stream, &dev)) return -1;
-    if (enqueue_passes<R>(st, plan, img, static_cast<char *>(dev), st->tmp, d_out)) return -1;
+    if (enqueue_passes<R>(st, plan, img, static_cast<char *>(dev), st->tmp, d_out, &fns)) return -1;
     if (nprobes > 0)
         HIP_TRY(hipMemcpyAsync(probes_out, d_out, sizeof(double) * nprobes, hipMemcpyDeviceToHost, st->stream));
     HIP_TRY(hipStreamSynchronize(st->stream));
@@ -940,13 +946,18 @@ int qsim_jit_compile_check(int num_qubits, int dtype, const qsim_op *ops, int no
                            uint64_t mat_count, int K, int cmin)
 {
     if (dtype != QSIM_C64 && dtype != QSIM_C128) return fail("unknown dtype");
-    if (check_ops(num_qubits, ops, nops, mat_count, false)) return -1;
+    if (check_ops(num_qubits, ops, nops, mat_count, true)) return -1;
     std::vector<PlanOp> pops = to_plan_ops(ops, nops);
+    int nv = 1;
+    for (int i = 0; i < nops; ++i)
+        if (ops[i].kind == QSIM_OP_PROBE) nv = 2;
     Plan plan;
     std::string err;
-    if (!plan_passes(num_qubits, pops.data(), nops, mats, (int)amp_bytes(dtype), K, cmin, 1, plan, err))
+    if (!plan_passes(num_qubits, pops.data(), nops, mats, (int)amp_bytes(dtype), K, cmin,
+                     nv == 2 ? kMaxProbes : 1, plan, err))
         return fail(err);
-    if (!jit_compile_only(plan, dtype == QSIM_C128, "gfx950", err)) return fail("run-time kernel compile: " + err);
+    if (!jit_compile_only(plan, dtype == QSIM_C128, nv, "gfx950", err))
+        return fail("run-time kernel compile: " + err);
     return (int)plan.passes.size();
 }
 

@@ -37,13 +37,14 @@ uint64_t fnv1a(const std::string &s)
     return h;
 }
 
-bool pass_has_probes(const Plan &plan, size_t p) { return plan.passes[p].nprobe > 0; }
+// Forward calls (one vector) never carry probes; the slot kernel keeps them.
+bool skip_pass(const Plan &plan, size_t p, int nv) { return nv == 1 && plan.passes[p].nprobe > 0; }
 
 // Pair slots of a round: register bits (first, second).
 constexpr int kPairF[6] = {0, 2, 0, 1, 0, 1};
 constexpr int kPairS[6] = {1, 3, 2, 3, 3, 2};
 
-void emit_step(std::ostringstream &o, const StepDev &sd, int E, const char *T)
+void emit_step(std::ostringstream &o, const StepDev &sd, int E, const char *T, int nv)
 {
     const int NG = E >> kG;
     int moff[16];
@@ -61,39 +62,67 @@ void emit_step(std::ostringstream &o, const StepDev &sd, int E, const char *T)
         o << "    const uint32_t g32 = (uint32_t)gi;\n";
     o << "    const int sb = swz((int)ins0(ins0(ins0(ins0(g32, " << sd.sbits[0] << "), " << sd.sbits[1] << "), "
       << sd.sbits[2] << "), " << sd.sbits[3] << "));\n";
-    o << "    cx<" << T << "> a[16];\n";
-    for (int m = 0; m < 16; ++m) o << "    a[" << m << "] = c0[sb ^ " << moff[m] << "];\n";
+    // vectors: a (psi, in c0) and, with nv == 2, l (lam, in c1)
+    const char *vec[2] = {"a", "l"}, *lds[2] = {"c0", "c1"};
+    for (int v = 0; v < nv; ++v) {
+        o << "    cx<" << T << "> " << vec[v] << "[16];\n";
+        for (int m = 0; m < 16; ++m) o << "    " << vec[v] << "[" << m << "] = " << lds[v] << "[sb ^ " << moff[m] << "];\n";
+    }
     const char *store_guard = guard ? "if (act) " : "";
     if (sd.dense >= 0) {
         if (sd.scale >= 0) o << "    const cx<" << T << "> f = mats[" << sd.scale << "];\n";
-        for (int r = 0; r < 16; ++r) {
-            const int M = sd.dense + 16 * r;
-            o << "    { cx<" << T << "> acc = cmul(mats[" << M << "], a[0]);\n";
-            o << "      #pragma unroll\n      for (int c = 1; c < 16; ++c) acc = cmad(mats[" << M
-              << " + c], a[c], acc);\n";
-            if (sd.scale >= 0) o << "      acc = cmul(acc, f);\n";
-            o << "      " << store_guard << "c0[sb ^ " << moff[r] << "] = acc; }\n";
-        }
+        for (int v = 0; v < nv; ++v)
+            for (int r = 0; r < 16; ++r) {
+                const int M = sd.dense + 16 * r;
+                o << "    { cx<" << T << "> acc = cmul(mats[" << M << "], " << vec[v] << "[0]);\n";
+                o << "      #pragma unroll\n      for (int c = 1; c < 16; ++c) acc = cmad(mats[" << M
+                  << " + c], " << vec[v] << "[c], acc);\n";
+                if (sd.scale >= 0) o << "      acc = cmul(acc, f);\n";
+                o << "      " << store_guard << lds[v] << "[sb ^ " << moff[r] << "] = acc; }\n";
+            }
     } else {
         for (int r = 0; r < sd.nrounds; ++r) {
             const int b = r * kSlotsPerRound;
             for (int j = 0; j < 4; ++j) {
                 const int k = sd.kind[b + j];
+                const int mo = sd.mat[b + j];
                 if (k == S1_SKIP) continue;
-                o << "    apply1<" << T << ", " << j << ">(a, " << k << ", mats + " << sd.mat[b + j] << ");\n";
+                if (k == S1_PROBE || k == S1_PROBE_GEN) {
+                    // Im<lam|G|psi> over this step's pairs, one partial per wave
+                    o << "    { const cx<" << T << "> g[4] = {mats[" << mo << "], mats[" << mo + 1 << "], mats["
+                      << mo + 2 << "], mats[" << mo + 3 << "]};\n";
+                    o << "      const double pv = wave_sum(" << (guard ? "act ? " : "") << "probe1<" << T << ", "
+                      << j << ">(a, l, g)" << (guard ? " : 0.0" : "") << ");\n";
+                    o << "      if ((threadIdx.x & 63) == 0) pacc[" << sd.probe[4 * r + j]
+                      << " * kWaves + (threadIdx.x >> 6)] += pv; }\n";
+                    if (k == S1_PROBE_GEN) {
+                        o << "    { const cx<" << T << "> u[4] = {mats[" << mo + 4 << "], mats[" << mo + 5
+                          << "], mats[" << mo + 6 << "], mats[" << mo + 7 << "]};\n";
+                        o << "      reg1<" << T << ", " << j << ">(a, u); reg1<" << T << ", " << j << ">(l, u); }\n";
+                    }
+                    continue;
+                }
+                for (int v = 0; v < nv; ++v)
+                    o << "    apply1<" << T << ", " << j << ">(" << vec[v] << ", " << k << ", mats + " << mo << ");\n";
             }
             for (int j = 0; j < 6; ++j) {
                 const int k = sd.kind[b + 4 + j];
                 if (k == S2_SKIP) continue;
-                o << "    apply2<" << T << ", " << kPairF[j] << ", " << kPairS[j] << ">(a, " << k << ", mats + "
-                  << sd.mat[b + 4 + j] << ");\n";
+                for (int v = 0; v < nv; ++v)
+                    o << "    apply2<" << T << ", " << kPairF[j] << ", " << kPairS[j] << ">(" << vec[v] << ", " << k
+                      << ", mats + " << sd.mat[b + 4 + j] << ");\n";
             }
         }
         if (sd.scale >= 0) {
             o << "    { const cx<" << T << "> f = mats[" << sd.scale << "];\n";
-            o << "      #pragma unroll\n      for (int m = 0; m < 16; ++m) a[m] = cmul(a[m], f); }\n";
+            for (int v = 0; v < nv; ++v)
+                o << "      #pragma unroll\n      for (int m = 0; m < 16; ++m) " << vec[v] << "[m] = cmul(" << vec[v]
+                  << "[m], f);\n";
+            o << "    }\n";
         }
-        for (int m = 0; m < 16; ++m) o << "    " << store_guard << "c0[sb ^ " << moff[m] << "] = a[" << m << "];\n";
+        for (int v = 0; v < nv; ++v)
+            for (int m = 0; m < 16; ++m)
+                o << "    " << store_guard << lds[v] << "[sb ^ " << moff[m] << "] = " << vec[v] << "[" << m << "];\n";
     }
     if (loop) o << "  }\n";
     o << "  }\n  __syncthreads();\n";
@@ -177,16 +206,24 @@ bool compile_all(const std::vector<std::string> &bodies, const std::string &arch
 
 }  // namespace
 
-std::string jit_pass_source(const Plan &plan, size_t p, bool dbl, const std::string &name)
+std::string jit_pass_source(const Plan &plan, size_t p, bool dbl, int nv, const std::string &name)
 {
     const PassDev &pd = plan.passes[p];
     const char *T = dbl ? "double" : "float";
     const int K = pd.K, CF = pd.cfast, E = 1 << K, NROW = 1 << (K - CF);
+    const int NP = nv == 2 ? std::max(1, pd.nprobe) : 0;
     std::ostringstream o;
     o << "extern \"C\" __global__ __launch_bounds__(kTPB)"
-      << (dbl ? "" : " __attribute__((amdgpu_waves_per_eu(4)))") << "\nvoid " << name << "(cx<" << T
-      << "> *__restrict__ psi, const cx<" << T << "> *__restrict__ mats)\n{\n";
+      << (dbl || nv == 2 ? "" : " __attribute__((amdgpu_waves_per_eu(4)))") << "\nvoid " << name << "(cx<" << T
+      << "> *__restrict__ psi, ";
+    if (nv == 2) o << "cx<" << T << "> *__restrict__ lam, double *__restrict__ probe_partial, ";
+    o << "const cx<" << T << "> *__restrict__ mats)\n{\n";
     o << "  __shared__ __align__(16) cx<" << T << "> c0[" << E << "];\n";
+    if (nv == 2) {
+        o << "  __shared__ __align__(16) cx<" << T << "> c1[" << E << "];\n";
+        o << "  __shared__ double pacc[" << NP << " * kWaves];\n";
+        o << "  for (int t = threadIdx.x; t < " << NP << " * kWaves; t += kTPB) pacc[t] = 0.0;\n";
+    }
     o << "  __shared__ uint64_t rowoff[" << NROW << "];\n";
     o << "  for (int t = threadIdx.x; t < " << NROW << "; t += kTPB) {\n    uint64_t r = 0;\n";
     for (int k = CF; k < K; ++k)
@@ -201,13 +238,23 @@ std::string jit_pass_source(const Plan &plan, size_t p, bool dbl, const std::str
     }
     o << "  __syncthreads();\n";
     o << "  chunk_copy<" << T << ", true>(psi, c0, base, rowoff, " << E << ", " << CF << ");\n";
+    if (nv == 2) o << "  chunk_copy<" << T << ", true>(lam, c1, base, rowoff, " << E << ", " << CF << ");\n";
     o << "  __syncthreads();\n";
-    for (int s = 0; s < pd.nsteps; ++s) emit_step(o, plan.steps[pd.step_off + s], E, T);
-    o << "  chunk_copy<" << T << ", false>(psi, c0, base, rowoff, " << E << ", " << CF << ");\n}\n\n";
+    for (int s = 0; s < pd.nsteps; ++s) emit_step(o, plan.steps[pd.step_off + s], E, T, nv);
+    o << "  chunk_copy<" << T << ", false>(psi, c0, base, rowoff, " << E << ", " << CF << ");\n";
+    if (nv == 2) {
+        o << "  chunk_copy<" << T << ", false>(lam, c1, base, rowoff, " << E << ", " << CF << ");\n";
+        if (pd.nprobe > 0) {
+            o << "  for (int q = threadIdx.x; q < " << pd.nprobe << "; q += kTPB) {\n";
+            o << "    double v = 0.0;\n    for (int w = 0; w < kWaves; ++w) v += pacc[q * kWaves + w];\n";
+            o << "    probe_partial[c * " << pd.nprobe << " + q] = v;\n  }\n";
+        }
+    }
+    o << "}\n\n";
     return o.str();
 }
 
-bool jit_functions(const Plan &plan, bool dbl, int device, const std::string &arch,
+bool jit_functions(const Plan &plan, bool dbl, int nv, int device, const std::string &arch,
                    std::vector<hipFunction_t> &out, std::string &err)
 {
     const size_t np = plan.passes.size();
@@ -219,15 +266,15 @@ bool jit_functions(const Plan &plan, bool dbl, int device, const std::string &ar
         std::lock_guard<std::mutex> lk(g_mu);
         std::map<std::string, int> seen;
         for (size_t p = 0; p < np; ++p) {
-            if (pass_has_probes(plan, p)) continue;
-            keys[p] = jit_pass_source(plan, p, dbl, "@");
+            if (skip_pass(plan, p, nv)) continue;
+            keys[p] = jit_pass_source(plan, p, dbl, nv, "@");
             auto it = g_fns.find({device, keys[p]});
             if (it != g_fns.end()) {
                 out[p] = it->second;
             } else if (!seen.count(keys[p])) {
                 seen[keys[p]] = 1;
                 missing_keys.push_back(keys[p]);
-                missing_bodies.push_back(jit_pass_source(plan, p, dbl, kernel_name(keys[p])));
+                missing_bodies.push_back(jit_pass_source(plan, p, dbl, nv, kernel_name(keys[p])));
             }
         }
     }
@@ -260,12 +307,12 @@ bool jit_functions(const Plan &plan, bool dbl, int device, const std::string &ar
     return true;
 }
 
-bool jit_compile_only(const Plan &plan, bool dbl, const std::string &arch, std::string &err)
+bool jit_compile_only(const Plan &plan, bool dbl, int nv, const std::string &arch, std::string &err)
 {
     std::vector<std::string> bodies;
     for (size_t p = 0; p < plan.passes.size(); ++p) {
-        if (pass_has_probes(plan, p)) continue;
-        bodies.push_back(jit_pass_source(plan, p, dbl, kernel_name(jit_pass_source(plan, p, dbl, "@"))));
+        if (skip_pass(plan, p, nv)) continue;
+        bodies.push_back(jit_pass_source(plan, p, dbl, nv, kernel_name(jit_pass_source(plan, p, dbl, nv, "@"))));
     }
     if (bodies.empty()) return true;
     std::vector<std::vector<char>> codes;

@@ -17,19 +17,23 @@
 
 namespace qsim {
 
-// One function per pass of `plan` (nullptr for passes that stay on
-// k_tile_pass: those with gradient probes).  Kernels are cached per device
-// and source for the life of the process; missing ones are compiled in
-// parallel.  `arch` is the device's gcnArchName.  Returns false with `err`.
-bool jit_functions(const Plan &plan, bool dbl, int device, const std::string &arch,
+// One function per pass of `plan` for `nv` vectors (1: psi, forward;
+// 2: psi and lam with gradient probes, the adjoint's reverse sweep).  nullptr
+// marks a pass left to k_tile_pass (probes in a one-vector call).  Kernels
+// are cached per device and source for the life of the process; missing ones
+// are compiled in parallel.  `arch` is the device's gcnArchName.  Returns
+// false with `err`.
+//   nv 1: kernel(psi, mats)
+//   nv 2: kernel(psi, lam, probe_partial, mats), probe_partial[block * nprobe + k]
+bool jit_functions(const Plan &plan, bool dbl, int nv, int device, const std::string &arch,
                    std::vector<hipFunction_t> &out, std::string &err);
 
 // Source of the specialised kernel for pass `p` (kernel name `name`).
-std::string jit_pass_source(const Plan &plan, size_t p, bool dbl, const std::string &name);
+std::string jit_pass_source(const Plan &plan, size_t p, bool dbl, int nv, const std::string &name);
 
 // Compile every pass of `plan` for `arch` without loading it (no device
 // needed).  Returns false with the compiler log in `err`.
-bool jit_compile_only(const Plan &plan, bool dbl, const std::string &arch, std::string &err);
+bool jit_compile_only(const Plan &plan, bool dbl, int nv, const std::string &arch, std::string &err);
 
 // Kernels compiled so far in this process (all devices).
 long jit_compiled_count();

@@ -159,9 +159,11 @@ int qsim_set_tile_params(qsim_state *st, int K, int cmin);
  * same order. */
 int qsim_set_jit(qsim_state *st, int enable);
 
-/* Host only, no reference counterpart: plan `ops` as qsim_apply_ops would and
- * compile (without loading) every specialised pass kernel for gfx950.
- * Returns the number of passes, or -1 with the compiler log. */
+/* Host only, no reference counterpart: plan `ops` as qsim_apply_ops would (as
+ * the adjoint's reverse sweep would, on psi and lam, when `ops` hold
+ * QSIM_OP_PROBE entries) and compile, without loading, every specialised pass
+ * kernel for gfx950.  Returns the number of passes, or -1 with the compiler
+ * log. */
 int qsim_jit_compile_check(int num_qubits, int dtype, const qsim_op *ops, int nops,
                            const void *mats, uint64_t mat_count, int K, int cmin);
 

@@ -288,3 +288,18 @@ def test_specialised_pass_kernels_compile(dtype, K, cmin):
     npass = hip.jit_compile_check(n, arr, len(ops), arena, K, cmin)
     passes, _ = hip.plan_describe(n, arr, len(ops), arena, K, cmin)
     assert npass == len(passes) > 0
+
+
+@pytest.mark.parametrize("use_float32", [True, False])
+def test_specialised_adjoint_kernels_compile(use_float32):
+    """The reverse sweep's two-vector pass kernels, with gradient probes,
+    compile for gfx950 (hiprtc, no device needed)."""
+    from frontierlabproject_amd.adjoint import reverse_ops
+    from frontierlabproject_amd.backends.custom.native import qgpusim_hip as hip
+    n, layers = 12, 3
+    theta = np.asarray(W.ansatz_params(n, layers))
+    buf, nprobe = reverse_ops(W.ansatz_spec(n, layers), theta, np.complex64 if use_float32 else np.complex128)
+    assert nprobe == 2 * n * layers
+    arena = buf.arena()
+    K, cmin = (12, 4) if use_float32 else (11, 3)
+    assert hip.jit_compile_check(n, buf.arr, buf.n, arena, K, cmin) > 0
