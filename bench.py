@@ -309,9 +309,9 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(chunk_gbps, 1), "peak": PEAK_HBM_GBPS,
                      "unit": "GB/s", "frac": round(chunk_gbps / PEAK_HBM_GBPS, 4),
                      "traffic": traffic and round(traffic["chunk_pass"]["traffic_bytes_per_launch"]),
-                     "kernel": "k_tile_pass<float,1> (LDS chunk pass)",
+                     "kernel": "qsim_jit_* (LDS chunk pass, one kernel specialised per pass)",
                      "launches": cp_n, "avg_ms": round(chunk_avg_ms, 4)},
-        "permute_kernel": {"achieved": round(perm_gbps, 1), "frac": round(perm_gbps / PEAK_HBM_GBPS, 4),
+        "permute_kernel": {"kernel": "k_permute<float>", "achieved": round(perm_gbps, 1), "frac": round(perm_gbps / PEAK_HBM_GBPS, 4),
                            "traffic": traffic and round(traffic["permute"]["traffic_bytes_per_launch"]),
                            "launches": pm_n, "avg_ms": round(perm_avg_ms, 4)},
         "norm_ok": tile_result_ok,

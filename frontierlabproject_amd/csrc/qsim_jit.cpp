@@ -229,7 +229,8 @@ std::string jit_pass_source(const Plan &plan, size_t p, bool dbl, int nv, const 
     for (int k = CF; k < K; ++k)
         o << "    r |= (uint64_t)((t >> " << k - CF << ") & 1) << " << pd.pos[k] << ";\n";
     o << "    rowoff[t] = r;\n  }\n";
-    o << "  const uint64_t c = blockIdx.x;\n  uint64_t base = 0;\n";
+    o << "  const uint64_t c = xcd_slab(blockIdx.x, " << pd.ncomp << ");\n";
+    o << "  uint64_t base = 0;\n";
     for (int m = 0; m < pd.ncomp;) {  // runs of consecutive positions move as one field
         int len = 1;
         while (m + len < pd.ncomp && pd.comp[m + len] == pd.comp[m] + len) ++len;
@@ -237,13 +238,14 @@ std::string jit_pass_source(const Plan &plan, size_t p, bool dbl, int nv, const 
         m += len;
     }
     o << "  __syncthreads();\n";
-    o << "  chunk_copy<" << T << ", true>(psi, c0, base, rowoff, " << E << ", " << CF << ");\n";
-    if (nv == 2) o << "  chunk_copy<" << T << ", true>(lam, c1, base, rowoff, " << E << ", " << CF << ");\n";
+    const std::string ntf = ", true";  // streamed once: non-temporal global accesses
+    o << "  chunk_copy<" << T << ", true" << ntf << ">(psi, c0, base, rowoff, " << E << ", " << CF << ");\n";
+    if (nv == 2) o << "  chunk_copy<" << T << ", true" << ntf << ">(lam, c1, base, rowoff, " << E << ", " << CF << ");\n";
     o << "  __syncthreads();\n";
     for (int s = 0; s < pd.nsteps; ++s) emit_step(o, plan.steps[pd.step_off + s], E, T, nv);
-    o << "  chunk_copy<" << T << ", false>(psi, c0, base, rowoff, " << E << ", " << CF << ");\n";
+    o << "  chunk_copy<" << T << ", false" << ntf << ">(psi, c0, base, rowoff, " << E << ", " << CF << ");\n";
     if (nv == 2) {
-        o << "  chunk_copy<" << T << ", false>(lam, c1, base, rowoff, " << E << ", " << CF << ");\n";
+        o << "  chunk_copy<" << T << ", false" << ntf << ">(lam, c1, base, rowoff, " << E << ", " << CF << ");\n";
         if (pd.nprobe > 0) {
             o << "  for (int q = threadIdx.x; q < " << pd.nprobe << "; q += kTPB) {\n";
             o << "    double v = 0.0;\n    for (int w = 0; w < kWaves; ++w) v += pacc[q * kWaves + w];\n";

@@ -262,7 +262,7 @@ hipError_t launch_diag1q(cx<R> *psi, int n, int q, cx<R> p0, cx<R> p1, hipStream
 // contiguous output amplitudes.  Tile slot r (input order) is stored at
 // r ^ ((r >> a) & (2^a - 1)) so that the output-order reads spread over banks.
 // ----------------------------------------------------------------------------
-template <typename R>
+template <typename R, bool NT>
 __global__ __launch_bounds__(kTPB) void k_permute(const cx<R> *__restrict__ in, cx<R> *__restrict__ out, PermArgs P)
 {
     extern __shared__ __align__(16) unsigned char smem[];
@@ -297,7 +297,7 @@ __global__ __launch_bounds__(kTPB) void k_permute(const cx<R> *__restrict__ in, 
             if ((t >> k) & 1) r |= 1u << P.w2r[k];
         w2r_lo[t] = r;
     }
-    const uint64_t g = blockIdx.x;
+    const uint64_t g = xcd_slab(blockIdx.x, P.ncomp);
     uint64_t base_in = 0, base_out = 0;
     for (int m = 0; m < P.ncomp; ++m)
         if ((g >> m) & 1) {
@@ -318,7 +318,7 @@ __global__ __launch_bounds__(kTPB) void k_permute(const cx<R> *__restrict__ in, 
             for (int j = 0; j < kBatch; ++j) {
                 const int v = v0 + j * kTPB + threadIdx.x;
                 const int r = v << 1;
-                if (v < nv) buf[j] = *reinterpret_cast<const v4f *>(in + base_in + row_in[r >> a] + (r & amask));
+                if (v < nv) buf[j] = gload<NT>(reinterpret_cast<const v4f *>(in + base_in + row_in[r >> a] + (r & amask)));
             }
 #pragma unroll
             for (int j = 0; j < kBatch; ++j) {
@@ -344,7 +344,7 @@ __global__ __launch_bounds__(kTPB) void k_permute(const cx<R> *__restrict__ in, 
             for (int j = 0; j < kBatch; ++j) {
                 const int v = v0 + j * kTPB + threadIdx.x;
                 const int w = v << 1;
-                if (v < nv) *reinterpret_cast<v4f *>(out + base_out + row_out[w >> b] + (w & bmask)) = buf[j];
+                if (v < nv) gstore<NT>(reinterpret_cast<v4f *>(out + base_out + row_out[w >> b] + (w & bmask)), buf[j]);
             }
         }
     } else {
@@ -357,7 +357,7 @@ __global__ __launch_bounds__(kTPB) void k_permute(const cx<R> *__restrict__ in, 
 #pragma unroll
             for (int j = 0; j < kBatch; ++j) {
                 const int r = r0 + j * kTPB + threadIdx.x;
-                if (r < S) buf[j] = ir[base_in + row_in[r >> a] + (r & amask)];
+                if (r < S) buf[j] = gload<NT>(ir + base_in + row_in[r >> a] + (r & amask));
             }
 #pragma unroll
             for (int j = 0; j < kBatch; ++j) {
@@ -376,7 +376,7 @@ __global__ __launch_bounds__(kTPB) void k_permute(const cx<R> *__restrict__ in, 
 #pragma unroll
             for (int j = 0; j < kBatch; ++j) {
                 const int w = w0 + j * kTPB + threadIdx.x;
-                if (w < S) ow[base_out + row_out[w >> b] + (w & bmask)] = buf[j];
+                if (w < S) gstore<NT>(ow + base_out + row_out[w >> b] + (w & bmask), buf[j]);
             }
         }
     }
@@ -394,7 +394,7 @@ hipError_t launch_permute(const cx<R> *in, cx<R> *out, const PermArgs &P, hipStr
 {
     const size_t sm = perm_smem<R>(P);
     const uint64_t blocks = 1ull << (P.n - P.s);
-    hipLaunchKernelGGL(k_permute<R>, dim3((unsigned)blocks), dim3(kTPB), sm, s, in, out, P);
+    hipLaunchKernelGGL((k_permute<R, true>), dim3((unsigned)blocks), dim3(kTPB), sm, s, in, out, P);
     return hipGetLastError();
 }
 
@@ -526,18 +526,18 @@ __attribute__((amdgpu_waves_per_eu((NV == 1 && sizeof(R) == 4) ? 4 : 1))) void k
     }
     __syncthreads();
 
-    const uint64_t c = blockIdx.x;
+    const uint64_t c = xcd_slab(blockIdx.x, pd->ncomp);
     if (c >= nchunks) return;
     const uint64_t base = chunk_base(c, pd);
-    chunk_copy<R, true>(psi, c0, base, rowoff, E, cf);
-    if (NV == 2) chunk_copy<R, true>(lam, c1, base, rowoff, E, cf);
+    chunk_copy<R, true, true>(psi, c0, base, rowoff, E, cf);
+    if (NV == 2) chunk_copy<R, true, true>(lam, c1, base, rowoff, E, cf);
     if (NV == 2)
         for (int t = threadIdx.x; t < nprobe * kWaves; t += kTPB) pacc[t] = 0.0;
     __syncthreads();
     run_steps<R, NV>(c0, c1, pd, steps, mats, pacc);
-    chunk_copy<R, false>(psi, c0, base, rowoff, E, cf);
+    chunk_copy<R, false, true>(psi, c0, base, rowoff, E, cf);
     if (NV == 2) {
-        chunk_copy<R, false>(lam, c1, base, rowoff, E, cf);
+        chunk_copy<R, false, true>(lam, c1, base, rowoff, E, cf);
         for (int p = threadIdx.x; p < nprobe; p += kTPB) {
             double v = 0.0;
             for (int w = 0; w < kWaves; ++w) v += pacc[p * kWaves + w];
@@ -709,13 +709,13 @@ __global__ __launch_bounds__(kTPB) void k_pauli_chunk(const cx<R> *__restrict__ 
             if ((t >> (k - cf)) & 1) o |= 1ull << pd->pos[k];
         rowoff[t] = o;
     }
-    const uint64_t g = blockIdx.x;
+    const uint64_t g = xcd_slab(blockIdx.x, pd->ncomp);
     uint64_t base = 0;
     for (int m = 0; m < pd->ncomp; ++m)
         if ((g >> m) & 1) base |= 1ull << pd->comp[m];
     __syncthreads();
     const int lowmask = (1 << cf) - 1;
-    chunk_copy<R, true>(const_cast<cx<R> *>(psi), c0, base, rowoff, E, cf);
+    chunk_copy<R, true, true>(const_cast<cx<R> *>(psi), c0, base, rowoff, E, cf);
     __syncthreads();
 
     NSum er, ei;
@@ -784,7 +784,7 @@ hipError_t prepare_kernels()
     const void *fns[] = {
         reinterpret_cast<const void *>(k_tile_pass<float, 1>), reinterpret_cast<const void *>(k_tile_pass<float, 2>),
         reinterpret_cast<const void *>(k_tile_pass<double, 1>), reinterpret_cast<const void *>(k_tile_pass<double, 2>),
-        reinterpret_cast<const void *>(k_permute<float>), reinterpret_cast<const void *>(k_permute<double>),
+        reinterpret_cast<const void *>(k_permute<float, true>), reinterpret_cast<const void *>(k_permute<double, true>),
         reinterpret_cast<const void *>(k_pauli_chunk<float>), reinterpret_cast<const void *>(k_pauli_chunk<double>),
     };
     for (const void *f : fns) hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
