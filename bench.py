@@ -54,20 +54,33 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--adj-n", type=int, default=24)
     ap.add_argument("--adj-layers", type=int, default=20)
+    ap.add_argument("--workload", choices=["random33", "qft30"], default="random33",
+                    help="random33: the headline random circuit (default); qft30: QFT on 30 qubits, "
+                         "complex64, nL=16, from a seeded Gaussian state, checked against an FP64 DFT")
+    ap.add_argument("--dtype", choices=["c64", "c128"], default="c64",
+                    help="random33 state precision (c128: 128 GiB state; plain, adjoint and CPU legs skipped)")
     ap.add_argument("--calib-gib", type=float, default=0.0,
                     help="first copy this many GiB device-to-device (a known-size kernel for "
                          "calibrating hardware byte counters)")
     return ap.parse_args()
 
 
-def build_program(hip, n, depth, nL, seed):
-    """LQR tile plan -> [("perm", old, new) | ("tile", arr, nops, arena)]."""
-    import numpy as np
+def build_program(hip, n, depth, nL, seed, dtype=None):
+    """LQR tile plan of the seeded random circuit ->
+    [("perm", old, new) | ("tile", arr, nops, arena)]."""
     from frontierlabproject_amd import workloads as W
+
+    qc = W.random_circuit(n, depth, seed, measure_marker=True)
+    return build_program_for(hip, qc, nL, dtype)
+
+
+def build_program_for(hip, qc, nL, dtype=None):
+    import numpy as np
     from frontierlabproject_amd.backends.custom.backend import build_tile_ops, fuse_tile_ops
     from frontierlabproject_amd.transpiler import make_custom_pm
 
-    qc = W.random_circuit(n, depth, seed, measure_marker=True)
+    dtype = dtype or np.complex64
+    n = qc.num_qubits
     pm, lqr = make_custom_pm(nL)
     t0 = time.perf_counter()
     tqc = pm.run(qc)
@@ -81,11 +94,11 @@ def build_program(hip, n, depth, nL, seed):
         if t.layout != layout:
             prog.append(("perm", list(layout), list(t.layout)))
             layout = t.layout
-        ops, ng = build_tile_ops(t, qmap, np.complex64)
+        ops, ng = build_tile_ops(t, qmap, dtype)
         ngates += ng
-        ops = fuse_tile_ops(ops, np.complex64)
+        ops = fuse_tile_ops(ops, dtype)
         if ops:
-            arr, arena = hip.pack_ops(ops, n, np.complex64)
+            arr, arena = hip.pack_ops(ops, n, dtype)
             prog.append(("tile", arr, len(ops), arena))
     # program-order op list for the plain path
     plain = []
@@ -109,8 +122,9 @@ def measured_traffic(n, depth):
         return json.load(f)["kernels"]
 
 
-def run_program(hip, sv, prog, mode):
-    sv.reset()
+def run_program(hip, sv, prog, mode, reset=True):
+    if reset:
+        sv.reset()
     for item in prog:
         if item[0] == "perm":
             sv.permute(item[1], item[2])
@@ -163,16 +177,22 @@ def main():
         del src, dst
         torch.cuda.empty_cache()
 
+    if args.workload == "qft30":
+        return main_qft(args, hip, np)
+    c128 = args.dtype == "c128"
+    cdt = np.complex128 if c128 else np.complex64
+    if c128:  # config 2 in complex128: the tile path, its kernels and the mirror echo
+        args.no_plain = args.no_adjoint = args.no_cpu = True
     n = args.n
-    bytes_per_pass = 2 * (1 << n) * 8
-    log(f"building program: random circuit n={n} depth={args.depth} nL={args.nL}")
-    qc, plan, prog, plain, ngates, t_tr = build_program(hip, n, args.depth, args.nL, args.seed)
+    bytes_per_pass = 2 * (1 << n) * (16 if c128 else 8)
+    log(f"building program: random circuit n={n} depth={args.depth} nL={args.nL} {args.dtype}")
+    qc, plan, prog, plain, ngates, t_tr = build_program(hip, n, args.depth, args.nL, args.seed, cdt)
     nperm = sum(1 for p in prog if p[0] == "perm")
     log(f"{len(plan)} tiles, {nperm} permutations, {ngates} gates, transpile {t_tr:.2f} s")
 
-    sv = hip.StatevectorF32(n)
+    sv = hip.Statevector(n) if c128 else hip.StatevectorF32(n)
     if args.K:
-        sv.set_tile_params(args.K, args.cmin or 4)
+        sv.set_tile_params(args.K, args.cmin or (3 if c128 else 4))
 
     def sync():
         sv.synchronize()
@@ -247,7 +267,7 @@ def main():
             idx = [inv.find_bit(q).index for q in ci.qubits]
             U = ci.operation.to_matrix()
             mops.append(("1q", idx[0], U) if len(idx) == 1 else ("2q", idx[0], idx[1], U))
-        arr, arena = hip.pack_ops(mops, n, np.complex64)
+        arr, arena = hip.pack_ops(mops, n, cdt)
         run_program(hip, sv, prog, hip.GRAPHED)
         sv.apply_packed(hip.COOPERATIVE, arr, len(mops), arena)
         a0 = sv.amplitude(0)
@@ -286,7 +306,7 @@ def main():
         log(f"cpu baseline: {cpu['value']} GB/s on {cpu['cores']} threads")
         extra["cpu_s_per_circuit_extrapolated"] = round(cpu.pop("s_per_gate") * ngates * 2 ** (n - 26), 1)
 
-    traffic = measured_traffic(n, args.depth)
+    traffic = None if c128 else measured_traffic(n, args.depth)
     out = {
         "metric": METRIC,
         "value": round(value, 1),
@@ -298,9 +318,9 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "c64",
+        "dtype": args.dtype,
         "data": "synthetic",
-        "config": {"workload": f"random circuit {n}q depth {args.depth} c64, LQR nL={args.nL}, "
+        "config": {"workload": f"random circuit {n}q depth {args.depth} {args.dtype}, LQR nL={args.nL}, "
                                "graphed LDS chunk passes + single-pass permutations",
                    "n_qubits": n, "depth": args.depth, "nL": args.nL, "seed": args.seed,
                    "tiles": len(plan), "permutations": nperm, "gates": ngates,
@@ -311,13 +331,74 @@ def main():
                      "traffic": traffic and round(traffic["chunk_pass"]["traffic_bytes_per_launch"]),
                      "kernel": "qsim_jit_* (LDS chunk pass, one kernel specialised per pass)",
                      "launches": cp_n, "avg_ms": round(chunk_avg_ms, 4)},
-        "permute_kernel": {"kernel": "k_permute<float>", "achieved": round(perm_gbps, 1), "frac": round(perm_gbps / PEAK_HBM_GBPS, 4),
+        "permute_kernel": {"kernel": "k_permute<double>" if c128 else "k_permute<float>", "achieved": round(perm_gbps, 1), "frac": round(perm_gbps / PEAK_HBM_GBPS, 4),
                            "traffic": traffic and round(traffic["permute"]["traffic_bytes_per_launch"]),
                            "launches": pm_n, "avg_ms": round(perm_avg_ms, 4)},
         "norm_ok": tile_result_ok,
         "cpu_baseline": cpu,
     }
     out.update(extra)
+    print(json.dumps(out), flush=True)
+
+
+def main_qft(args, hip, np):
+    """QFT on 30 qubits, complex64 (8 GiB), LQR nL=16, from a seeded normalised
+    complex Gaussian state; controlled-phase angles pi/2^k in FP64.  Reports the
+    tile path's s/circuit, the chunk-pass and permutation kernels' GB/s, and
+    1 - F and the max-abs amplitude error (global phase aligned) against the
+    FP64 DFT of the input (the circuit is sqrt(N) * inverse DFT)."""
+    import scipy.fft
+    from frontierlabproject_amd import workloads as W
+
+    n, nL, seed = 30, 16, 1001
+    bytes_per_pass = 2 * (1 << n) * 8
+    qc = W.qft_circuit(n)
+    qc.save_statevector()  # final all-qubit tile: output in standard qubit order
+    _, plan, prog, _, ngates, t_tr = build_program_for(hip, qc, nL)
+    nperm = sum(1 for p in prog if p[0] == "perm")
+    log(f"qft {n}q: {len(plan)} tiles, {nperm} permutations, {ngates} gates")
+    psi0 = W.random_state(n, seed)
+    sv = hip.StatevectorF32(n)
+    sv.from_numpy(psi0.astype(np.complex64))
+    run_program(hip, sv, prog, hip.GRAPHED, reset=False)
+    got = sv.to_numpy()
+    for _ in range(args.warmup):
+        run_program(hip, sv, prog, hip.GRAPHED, reset=False)
+    sv.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        run_program(hip, sv, prog, hip.GRAPHED, reset=False)
+    sv.synchronize()
+    s_per_circuit = (time.perf_counter() - t0) / args.steps
+    sv.profile(True)
+    run_program(hip, sv, prog, hip.COOPERATIVE, reset=False)
+    prof = sv.profile_read()
+    sv.profile(False)
+    cp_ms, cp_n = prof["chunk_pass"]
+    pm_ms, pm_n = prof["permute"]
+    del sv
+    log("FP64 reference: inverse DFT of the input (scipy.fft)")
+    ref = scipy.fft.ifft(psi0, workers=min(16, os.cpu_count() or 1), norm="ortho")
+    del psi0
+    got = got.astype(np.complex128)  # every reduction below in FP64
+    ov = np.vdot(ref, got)
+    fid = abs(ov) ** 2 / (np.vdot(got, got).real * np.vdot(ref, ref).real)
+    ph = ov / abs(ov)
+    max_abs = float(np.max(np.abs(ref * ph - got)))
+    max_ref = float(np.max(np.abs(ref)))
+    chunk_gbps = bytes_per_pass * cp_n / (cp_ms * 1e-3) / 1e9 if cp_n else 0.0
+    perm_gbps = bytes_per_pass * pm_n / (pm_ms * 1e-3) / 1e9 if pm_n else 0.0
+    out = {"workload": f"QFT {n}q c64 (30 H, 435 CP(pi/2^k), 15 SWAP), LQR nL={nL}, graphed chunk passes, "
+                       f"seeded Gaussian input (seed {seed})",
+           "s_per_circuit": round(s_per_circuit, 5), "tiles": len(plan), "permutations": nperm, "gates": ngates,
+           "chunk_pass": {"launches": cp_n, "avg_ms": round(cp_ms / max(1, cp_n), 4), "GBps": round(chunk_gbps, 1),
+                          "frac": round(chunk_gbps / PEAK_HBM_GBPS, 4)},
+           "permute": {"launches": pm_n, "avg_ms": round(pm_ms / max(1, pm_n), 4), "GBps": round(perm_gbps, 1),
+                       "frac": round(perm_gbps / PEAK_HBM_GBPS, 4)},
+           "one_minus_fidelity_vs_fp64_dft": float(f"{1 - fid:.3e}"),
+           "max_abs_err_phase_aligned": float(f"{max_abs:.3e}"),
+           "max_abs_err_over_max_amplitude": float(f"{max_abs / max_ref:.3e}"),
+           "dtype": "c64", "steps": args.steps, "warmup": args.warmup}
     print(json.dumps(out), flush=True)
 
 
