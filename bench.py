@@ -1,6 +1,11 @@
 #!/usr/bin/env python3
 """Benchmark of the statevector tile path on one MI355X.
 
+Other workloads (--workload): qft30 (QFT, 30 qubits, complex64, nL=16, against
+an FP64 DFT) and batch4096 (4096 circuits of 12-22 qubits, <Z_0> and adjoint
+gradients in one persistent launch); --dtype c128 runs the headline circuit in
+complex128.
+
 Workload (one "step" = one circuit): the seeded random circuit of 33 qubits,
 depth 30 (per layer a gate from {H,X,Y,Z,S,T,RX,RY,RZ} on every qubit, then CX
 on the pairs of a random permutation with p = 0.5), complex64 (64 GiB state),
@@ -11,6 +16,8 @@ resident in HBM; the timed region is reset + all permutations + all tiles.
 Reported beside it, in the same run:
 * the plain per-gate path (one coalesced gate-stream kernel per gate, program
   order) on the same GPU: s/circuit and the state overlap with the tile path;
+* the fused path: the same tile plan with every register step fused into one
+  4-qubit unitary applied as a complex GEMM on the FP32 matrix cores;
 * 1 - fidelity of the tile path from a mirror circuit (U then U^dag, FP64 norm);
 * the adjoint-gradient / forward time ratio of the 24-qubit, 20-layer
   RY-RZ-CZ ansatz (960 parameters), complex64 with FP64 expectation;
@@ -51,12 +58,18 @@ def parse():
     ap.add_argument("--no-mirror", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fused", action="store_true",
+                    help="skip the fused leg (4-qubit unitaries as complex GEMMs on the matrix cores)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--adj-n", type=int, default=24)
     ap.add_argument("--adj-layers", type=int, default=20)
-    ap.add_argument("--workload", choices=["random33", "qft30"], default="random33",
+    ap.add_argument("--workload", choices=["random33", "qft30", "batch4096"], default="random33",
                     help="random33: the headline random circuit (default); qft30: QFT on 30 qubits, "
-                         "complex64, nL=16, from a seeded Gaussian state, checked against an FP64 DFT")
+                         "complex64, nL=16, from a seeded Gaussian state, checked against an FP64 DFT; "
+                         "batch4096: 4096 random circuits of 12-22 qubits, <Z_0> and adjoint gradients in "
+                         "one persistent launch")
+    ap.add_argument("--autotune-cache", default="",
+                    help="batch4096: autotuner cache file (default: the library's per-user cache)")
     ap.add_argument("--dtype", choices=["c64", "c128"], default="c64",
                     help="random33 state precision (c128: 128 GiB state; plain, adjoint and CPU legs skipped)")
     ap.add_argument("--calib-gib", type=float, default=0.0,
@@ -179,6 +192,8 @@ def main():
 
     if args.workload == "qft30":
         return main_qft(args, hip, np)
+    if args.workload == "batch4096":
+        return main_batch(args, hip, np)
     c128 = args.dtype == "c128"
     cdt = np.complex128 if c128 else np.complex64
     if c128:  # config 2 in complex128: the tile path, its kernels and the mirror echo
@@ -249,6 +264,9 @@ def main():
         run_program(hip, sv, prog, hip.GRAPHED)
         ov = sv.overlap(sp)
         fid_tp = abs(ov) ** 2 / (sv.norm_squared() * sp.norm_squared())
+        if not args.no_fused and not c128:
+            extra["fused"] = fused_leg(hip, sv, prog, args, bytes_per_pass, sp)
+            sv.set_fusion(0)
         extra.update({
             "plain_s_per_circuit": round(t_plain, 4),
             "speedup_vs_plain": round(t_plain / s_per_circuit, 3),
@@ -341,6 +359,32 @@ def main():
     print(json.dumps(out), flush=True)
 
 
+def fused_leg(hip, sv, prog, args, bytes_per_pass, ref_state):
+    """The same program with every register step fused into one 4-qubit
+    unitary applied as a complex GEMM on the FP32 matrix cores (3M)."""
+    sv.set_fusion(1)
+    for _ in range(max(1, args.warmup)):
+        run_program(hip, sv, prog, hip.GRAPHED)
+    sv.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        run_program(hip, sv, prog, hip.GRAPHED)
+    sv.synchronize()
+    s_fused = (time.perf_counter() - t0) / args.steps
+    ov = sv.overlap(ref_state)
+    fid = abs(ov) ** 2 / (sv.norm_squared() * ref_state.norm_squared())
+    sv.profile(True)
+    run_program(hip, sv, prog, hip.COOPERATIVE)
+    cp_ms, cp_n = sv.profile_read()["chunk_pass"]
+    sv.profile(False)
+    gbps = bytes_per_pass * cp_n / (cp_ms * 1e-3) / 1e9 if cp_n else 0.0
+    log(f"fused: {s_fused:.4f} s/circuit, chunk pass {gbps:.0f} GB/s, 1-F(fused, plain) = {1 - fid:.3e}")
+    return {"s_per_circuit": round(s_fused, 4), "chunk_passes": cp_n,
+            "chunk_pass_avg_ms": round(cp_ms / max(1, cp_n), 4), "chunk_pass_GBps": round(gbps, 1),
+            "chunk_pass_frac": round(gbps / PEAK_HBM_GBPS, 4),
+            "one_minus_fidelity_vs_plain": float(f"{1 - fid:.3e}")}
+
+
 def main_qft(args, hip, np):
     """QFT on 30 qubits, complex64 (8 GiB), LQR nL=16, from a seeded normalised
     complex Gaussian state; controlled-phase angles pi/2^k in FP64.  Reports the
@@ -398,6 +442,79 @@ def main_qft(args, hip, np):
            "one_minus_fidelity_vs_fp64_dft": float(f"{1 - fid:.3e}"),
            "max_abs_err_phase_aligned": float(f"{max_abs:.3e}"),
            "max_abs_err_over_max_amplitude": float(f"{max_abs / max_ref:.3e}"),
+           "dtype": "c64", "steps": args.steps, "warmup": args.warmup}
+    print(json.dumps(out), flush=True)
+
+
+def main_batch(args, hip, np):
+    """4096 circuits of the random family (qubits uniform in 12..22, depth in
+    10..40, per-circuit seeds), complex64, each returning <Z_0> and its
+    adjoint gradient over its rotation angles, from one persistent launch
+    replayed as a hipGraph.  Chunk bits per qubit count come from the
+    autotuner (cached)."""
+    from frontierlabproject_amd import batch as B
+    from frontierlabproject_amd import workloads as W
+
+    spec = W.batch_spec()
+    t0 = time.perf_counter()
+    tuning = B.autotune(spec, cache_path=args.autotune_cache or None, log=log)
+    t_tune = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    circuits = B.random_batch(spec)
+    packed = B.pack(circuits)
+    t_pack = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    sim = B.BatchSimulator(None, tuning=tuning, packed=packed)
+    t_load = time.perf_counter() - t0
+    info = sim.info()
+    log(f"batch: {info}, tune {t_tune:.1f} s, pack {t_pack:.1f} s, plan+upload {t_load:.1f} s")
+    for _ in range(max(1, args.warmup)):
+        sim.run(use_graph=True)
+    totals, walls, lats = [], [], None
+    for _ in range(args.steps):
+        w0 = time.perf_counter()
+        e, g, lat, tot = sim.run(use_graph=True)
+        walls.append(time.perf_counter() - w0)
+        totals.append(tot)
+        lats = lat
+    ncirc = len(spec)
+    t_launch = float(np.median(totals)) / 1e3
+    # sample check against single-circuit adjoint runs
+    from frontierlabproject_amd.adjoint import AdjointSimulator
+    order = sorted(range(ncirc), key=lambda i: spec[i][0])
+    sample = sorted(set(order[:: max(1, ncirc // 12)] + [order[-1]]))
+    max_de = max_dg = 0.0
+    t_single = 0.0
+    for i in sample:
+        n, gates = circuits[i]
+        sp, th = [], []
+        for name, qs, a in gates:
+            sp.append((name, qs, None if a is None else len(th)))
+            if a is not None:
+                th.append(a)
+        ref = AdjointSimulator(n, sp, [(0, 1, 1.0)], use_float32=True)
+        ref.value_and_grad(np.asarray(th))
+        w0 = time.perf_counter()
+        e_ref, g_ref = ref.value_and_grad(np.asarray(th))
+        t_single += time.perf_counter() - w0
+        max_de = max(max_de, abs(e[i] - e_ref))
+        if len(g_ref):
+            max_dg = max(max_dg, float(np.max(np.abs(g[i] - g_ref))))
+    ns = np.array([s[0] for s in spec])
+    out = {"workload": "4096 random circuits (12-22 qubits, depth 10-40, config-0 gate family, per-circuit seeds), "
+                       "c64, <Z_0> + adjoint gradient each, one persistent launch, hipGraph replay",
+           "circuits_per_s": round(ncirc / t_launch, 1), "batch_ms": round(t_launch * 1e3, 3),
+           "batch_wall_ms": round(float(np.median(walls)) * 1e3, 3),
+           "latency_ms_p50": round(float(np.percentile(lats, 50)), 4),
+           "latency_ms_p99": round(float(np.percentile(lats, 99)), 4),
+           "latency_ms_p99_by_n": {str(n): round(float(np.percentile(lats[ns == n], 99)), 4)
+                                   for n in sorted(set(ns.tolist()))},
+           "gradients": int(info["probes"]), "passes": int(info["passes"]), "workgroups": int(info["workgroups"]),
+           "tuning_K_by_n": tuning["K_by_n"],
+           "sample_vs_single_circuit": {"circuits": len(sample), "max_abs_dE": float(f"{max_de:.3e}"),
+                                        "max_abs_dgrad": float(f"{max_dg:.3e}"),
+                                        "single_path_s_per_circuit": round(t_single / len(sample), 5)},
+           "setup_s": {"autotune": round(t_tune, 2), "pack": round(t_pack, 2), "plan_upload": round(t_load, 2)},
            "dtype": "c64", "steps": args.steps, "warmup": args.warmup}
     print(json.dumps(out), flush=True)
 

@@ -77,7 +77,14 @@ _SIGNATURES = {
     "qsim_profile_read": ([_P, _P, _P, _I], _I),
     "qsim_plan_describe": ([_I, _I, _P, _I, _P, _U64, _I, _I, _P, _I, _P], _I),
     "qsim_set_jit": ([_P, _I], _I),
-    "qsim_jit_compile_check": ([_I, _I, _P, _I, _P, _U64, _I, _I], _I),
+    "qsim_batch_create": ([ctypes.POINTER(_P)], _I),
+    "qsim_batch_destroy": ([_P], _I),
+    "qsim_batch_set_tuning": ([_P, _P, _I, _I], _I),
+    "qsim_batch_load": ([_P, _I, _P, _P, _P, _P, _P, _P, _P, _U64, _P], _I),
+    "qsim_batch_run": ([_P, _I, _P, _P, _P, _P], _I),
+    "qsim_batch_info": ([_P, _P, _I], _I),
+    "qsim_jit_compile_check": ([_I, _I, _P, _I, _P, _U64, _I, _I, _I], _I),
+    "qsim_set_fusion": ([_P, _I], _I),
 }
 
 EXPORTED = tuple(_SIGNATURES)
@@ -323,6 +330,11 @@ class _StatevectorBase:
     def set_tile_params(self, K, cmin):
         _check(lib.qsim_set_tile_params(self._h, int(K), int(cmin)))
 
+    def set_fusion(self, mode):
+        """0: slot programs; 1: every register step one fused 4-qubit unitary on
+        the matrix cores (complex64, specialised kernels)."""
+        _check(lib.qsim_set_fusion(self._h, int(mode)))
+
     def set_jit(self, enable=True):
         """Run forward chunk passes with kernels specialised per pass (hiprtc,
         compiled on first use and cached); False: the compiled slot-program kernel."""
@@ -425,12 +437,12 @@ def validate_ops(ops, num_qubits):
     _check(lib.qsim_validate_ops(num_qubits, ctypes.addressof(arr), len(ops), arena.size))
 
 
-def jit_compile_check(num_qubits, arr, nops, arena, K, cmin):
+def jit_compile_check(num_qubits, arr, nops, arena, K, cmin, fuse=False):
     """Host-only: plan packed ops and compile every specialised pass kernel for
     gfx950 without loading it; returns the number of passes."""
     code = C128 if arena.dtype == np.complex128 else C64
     npass = lib.qsim_jit_compile_check(num_qubits, code, ctypes.addressof(arr), nops, arena.ctypes.data,
-                                       arena.size, K, cmin)
+                                       arena.size, K, cmin, 1 if fuse else 0)
     if npass < 0:
         raise RuntimeError(lib.qsim_last_error().decode())
     return npass

@@ -108,6 +108,8 @@ size_t amp_bytes(int dtype) { return dtype == QSIM_C64 ? 8 : 16; }
 
 }  // namespace
 
+int qsim::set_error(const std::string &msg) { return fail(msg); }
+
 struct qsim_state {
     int n = 0;
     int dtype = QSIM_C64;
@@ -127,6 +129,7 @@ struct qsim_state {
     // permutations, chunk passes run by specialised kernels
     int64_t counters[6] = {0, 0, 0, 0, 0, 0};
     bool jit = true;               // specialise forward chunk passes at run time
+    bool fuse = false;             // every register step one 16x16 unitary (matrix cores)
     std::string arch;              // device gcnArchName (for run-time compiles)
     // per-kernel-class timing with events on `stream` (off by default)
     bool prof = false, capturing = false;
@@ -315,7 +318,7 @@ int plan_for(qsim_state *st, const qsim_op *ops, int nops, const void *mats, int
     std::vector<PlanOp> pops = to_plan_ops(ops, nops);
     std::string err;
     if (!plan_passes(st->n, pops.data(), nops, mats, (int)amp_bytes(st->dtype), st->K, st->cmin, max_probes, plan,
-                     err))
+                     err, st->fuse && max_probes == 1))
         return fail(err);
     return 0;
 }
@@ -364,6 +367,7 @@ int run_graphed(qsim_state *st, const qsim_op *ops, int nops, const void *mats, 
     put(&st->K, sizeof(int));
     put(&st->cmin, sizeof(int));
     put(&st->jit, sizeof(bool));
+    put(&st->fuse, sizeof(bool));
     put(ops, sizeof(qsim_op) * nops);
     put(mats, sizeof(cx<R>) * mat_count);
     for (auto it = st->graphs.begin(); it != st->graphs.end(); ++it) {
@@ -942,8 +946,16 @@ int qsim_set_jit(qsim_state *st, int enable)
     return 0;
 }
 
+int qsim_set_fusion(qsim_state *st, int mode)
+{
+    if (!st) return fail("null state");
+    if (mode != 0 && mode != 1) return fail("fusion mode must be 0 or 1");
+    st->fuse = mode == 1;
+    return 0;
+}
+
 int qsim_jit_compile_check(int num_qubits, int dtype, const qsim_op *ops, int nops, const void *mats,
-                           uint64_t mat_count, int K, int cmin)
+                           uint64_t mat_count, int K, int cmin, int fuse)
 {
     if (dtype != QSIM_C64 && dtype != QSIM_C128) return fail("unknown dtype");
     if (check_ops(num_qubits, ops, nops, mat_count, true)) return -1;
@@ -954,7 +966,7 @@ int qsim_jit_compile_check(int num_qubits, int dtype, const qsim_op *ops, int no
     Plan plan;
     std::string err;
     if (!plan_passes(num_qubits, pops.data(), nops, mats, (int)amp_bytes(dtype), K, cmin,
-                     nv == 2 ? kMaxProbes : 1, plan, err))
+                     nv == 2 ? kMaxProbes : 1, plan, err, fuse != 0 && nv == 1))
         return fail(err);
     if (!jit_compile_only(plan, dtype == QSIM_C128, nv, "gfx950", err))
         return fail("run-time kernel compile: " + err);

@@ -421,4 +421,187 @@ __device__ __forceinline__ uint64_t chunk_base(uint64_t c, const PassDev *__rest
     return base;
 }
 
+// The register steps of one pass over the chunk staged in LDS (c0, and c1 for
+// the second vector).  Ends with a barrier after every step.
+template <typename R, int NV>
+__device__ __forceinline__ void run_steps(cx<R> *c0, cx<R> *c1, const PassDev *__restrict__ pd,
+                                          const StepDev *__restrict__ steps, const cx<R> *__restrict__ mats,
+                                          double *pacc)
+{
+    const int E = 1 << pd->K;
+    const int ns = pd->nsteps;
+    const StepDev *st = steps + pd->step_off;
+    const int ngroups = E >> kG;
+    for (int si = 0; si < ns; ++si) {
+        const StepDev *sd = st + si;
+        const int r0 = sd->bits[0], r1 = sd->bits[1], r2 = sd->bits[2], r3 = sd->bits[3];
+        const int s0 = sd->sbits[0], s1 = sd->sbits[1], s2 = sd->sbits[2], s3 = sd->sbits[3];
+        const int nr = sd->nrounds, dn = sd->dense, sc = sd->scale;
+        // swz is XOR-linear and the group base and register offsets occupy
+        // disjoint bits, so each element's LDS slot is swz(base) ^ swz(offset):
+        // one XOR per element with a wave-uniform constant.
+        int moff[16];
+#pragma unroll
+        for (int m = 0; m < 16; ++m)
+            moff[m] = swz(((m & 1) << r0) | (((m >> 1) & 1) << r1) | (((m >> 2) & 1) << r2) |
+                          (((m >> 3) & 1) << r3));
+        // wave-uniform trip count: probes reduce across all 64 lanes
+        for (int gb = 0; gb < ngroups; gb += kTPB) {
+            const int gi = gb + threadIdx.x;
+            const bool act = gi < ngroups;
+            // idle lanes read a valid group and never store
+            const uint32_t g32 = (uint32_t)(act ? gi : ngroups - 1);
+            const int sb = swz((int)ins0(ins0(ins0(ins0(g32, s0), s1), s2), s3));
+            cx<R> a[16], l[NV == 2 ? 16 : 1];
+#pragma unroll
+            for (int m = 0; m < 16; ++m) {
+                a[m] = c0[sb ^ moff[m]];
+                if (NV == 2) l[m] = c1[sb ^ moff[m]];
+            }
+            if (dn >= 0) {
+                // rows go straight back to LDS: no second 16-amplitude array
+                const cx<R> f = sc >= 0 ? mats[sc] : cx<R>{R(1), R(0)};
+#pragma unroll
+                for (int r = 0; r < 16; ++r) {
+                    const cx<R> *M = mats + dn + 16 * r;
+                    cx<R> acc = cmul(M[0], a[0]);
+#pragma unroll
+                    for (int c = 1; c < 16; ++c) acc = cmad(M[c], a[c], acc);
+                    if (sc >= 0) acc = cmul(acc, f);
+                    if (act) c0[sb ^ moff[r]] = acc;
+                }
+                if (NV == 2) {
+#pragma unroll
+                    for (int r = 0; r < 16; ++r) {
+                        const cx<R> *M = mats + dn + 16 * r;
+                        cx<R> acc = cmul(M[0], l[0]);
+#pragma unroll
+                        for (int c = 1; c < 16; ++c) acc = cmad(M[c], l[c], acc);
+                        if (sc >= 0) acc = cmul(acc, f);
+                        if (act) c1[sb ^ moff[r]] = acc;
+                    }
+                }
+                continue;
+            } else {
+                // one copy of the round code, run nrounds times
+#pragma unroll 1
+                for (int r = 0; r < nr; ++r) {
+                    const int b = r * kSlotsPerRound;
+                    slot1<R, NV, 0>(a, l, sd->kind[b + 0], mats + sd->mat[b + 0], sd->probe[4 * r + 0], pacc, act);
+                    slot1<R, NV, 1>(a, l, sd->kind[b + 1], mats + sd->mat[b + 1], sd->probe[4 * r + 1], pacc, act);
+                    slot1<R, NV, 2>(a, l, sd->kind[b + 2], mats + sd->mat[b + 2], sd->probe[4 * r + 2], pacc, act);
+                    slot1<R, NV, 3>(a, l, sd->kind[b + 3], mats + sd->mat[b + 3], sd->probe[4 * r + 3], pacc, act);
+                    slot2<R, NV, 0, 1>(a, l, sd->kind[b + 4], mats + sd->mat[b + 4]);
+                    slot2<R, NV, 2, 3>(a, l, sd->kind[b + 5], mats + sd->mat[b + 5]);
+                    slot2<R, NV, 0, 2>(a, l, sd->kind[b + 6], mats + sd->mat[b + 6]);
+                    slot2<R, NV, 1, 3>(a, l, sd->kind[b + 7], mats + sd->mat[b + 7]);
+                    slot2<R, NV, 0, 3>(a, l, sd->kind[b + 8], mats + sd->mat[b + 8]);
+                    slot2<R, NV, 1, 2>(a, l, sd->kind[b + 9], mats + sd->mat[b + 9]);
+                }
+            }
+            if (sc >= 0) {  // scalar factored out of earlier gates
+                const cx<R> f = mats[sc];
+#pragma unroll
+                for (int m = 0; m < 16; ++m) {
+                    a[m] = cmul(a[m], f);
+                    if (NV == 2) l[m] = cmul(l[m], f);
+                }
+            }
+            if (act) {
+#pragma unroll
+                for (int m = 0; m < 16; ++m) {
+                    c0[sb ^ moff[m]] = a[m];
+                    if (NV == 2) c1[sb ^ moff[m]] = l[m];
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+
+// Dense 16x16 step of a complex64 chunk on the matrix cores.  The wave's 64
+// register groups are the columns of X (16 x 64, row = register index m), and
+// X <- M X runs as 16-column blocks of three real 16x16x4 FP32 MFMA products
+// each (3M): P1 = Mr Xr, P2 = Mi Xi, P3 = (Mr + Mi)(Xr + Xi);
+// Re = P1 - P2, Im = P3 - P1 - P2, FP32 accumulation.
+// v_mfma_f32_16x16x4f32 lane layout: A[i][k] from lane i + 16 k, B[k][j] from
+// lane j + 16 k, D[4 q + r][j] in lane j + 16 q, element r.
+// Lane j + 16 q therefore reads rows 4 kb + q (kb = 0..3) of group
+// (block base + j) and writes rows 4 q + r of the same group: every group is
+// read and written by one lane set of one wave, in program order.
+// moff[m]: LDS slot offset of register index m; s0..s3: the step's register
+// bits ascending; ng: register groups in the chunk; sc: optional factor.
+__device__ __forceinline__ void dense16_mfma(cx<float> *c0, const cx<float> *__restrict__ M, const int *moff,
+                                             int s0, int s1, int s2, int s3, int ng, const cx<float> *sc)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int j = lane & 15, q = lane >> 4;
+    float ar[4], ai[4], as[4];
+#pragma unroll
+    for (int kb = 0; kb < 4; ++kb) {
+        const cx<float> m = M[16 * j + 4 * kb + q];
+        ar[kb] = m.re;
+        ai[kb] = m.im;
+        as[kb] = m.re + m.im;
+    }
+    const cx<float> f = sc ? *sc : cx<float>{1.f, 0.f};
+    for (int gb = wave * 64; gb < ng; gb += kTPB) {  // wave-uniform trip count
+#pragma unroll
+        for (int nb = 0; nb < 4; ++nb) {
+            const int g = gb + 16 * nb + j;
+            const bool act = g < ng;
+            const int sb = swz((int)ins0(ins0(ins0(ins0((uint32_t)(act ? g : ng - 1), s0), s1), s2), s3));
+            float xr[4], xi[4];
+#pragma unroll
+            for (int kb = 0; kb < 4; ++kb) {
+                const cx<float> v = c0[sb ^ moff[4 * kb + q]];
+                xr[kb] = v.re;
+                xi[kb] = v.im;
+            }
+            v4f p1 = {0.f, 0.f, 0.f, 0.f}, p2 = p1, p3 = p1;
+#pragma unroll
+            for (int kb = 0; kb < 4; ++kb) {
+                p1 = __builtin_amdgcn_mfma_f32_16x16x4f32(ar[kb], xr[kb], p1, 0, 0, 0);
+                p2 = __builtin_amdgcn_mfma_f32_16x16x4f32(ai[kb], xi[kb], p2, 0, 0, 0);
+                p3 = __builtin_amdgcn_mfma_f32_16x16x4f32(as[kb], xr[kb] + xi[kb], p3, 0, 0, 0);
+            }
+            if (act) {
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    cx<float> o{p1[r] - p2[r], p3[r] - p1[r] - p2[r]};
+                    if (sc) o = cmul(o, f);
+                    c0[sb ^ moff[4 * q + r]] = o;
+                }
+            }
+        }
+    }
+}
+
+// FP64 reductions with Neumaier-compensated per-thread sums; fixed order,
+// so results are deterministic.
+struct NSum {
+    double s = 0.0, c = 0.0;
+    __device__ __forceinline__ void add(double x)
+    {
+        const double t = s + x;
+        c += (fabs(s) >= fabs(x)) ? ((s - t) + x) : ((x - t) + s);
+        s = t;
+    }
+    __device__ __forceinline__ double value() const { return s + c; }
+};
+
+__device__ __forceinline__ double block_sum(double v, double *sh)
+{
+    v = wave_sum(v);
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    __syncthreads();
+    if (lane == 0) sh[wave] = v;
+    __syncthreads();
+    double t = 0.0;
+    if (threadIdx.x == 0)
+        for (int w = 0; w < kWaves; ++w) t += sh[w];
+    return t;  // valid in thread 0
+}
+
 }  // namespace qsim

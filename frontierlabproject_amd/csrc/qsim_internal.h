@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <string>
 
 #include "qsim_plan.h"
 #include "qsim_device.h"
@@ -71,5 +72,8 @@ hipError_t launch_scale(cx<R> *psi, uint64_t dim, double factor, hipStream_t s);
 
 int max_dot_blocks();
 hipError_t prepare_kernels();
+
+// Sets the thread's last-error string (qsim_last_error) and returns -1.
+int set_error(const std::string &msg);
 
 }  // namespace qsim

@@ -52,6 +52,15 @@ void emit_step(std::ostringstream &o, const StepDev &sd, int E, const char *T, i
         moff[m] = swz_host(((m & 1) << sd.bits[0]) | (((m >> 1) & 1) << sd.bits[1]) |
                            (((m >> 2) & 1) << sd.bits[2]) | (((m >> 3) & 1) << sd.bits[3]));
     const bool loop = NG > kTPB, guard = NG < kTPB;
+    if (sd.dense >= 0 && nv == 1 && std::string(T) == "float") {  // matrix cores
+        o << "  { const int moff[16] = {";
+        for (int m = 0; m < 16; ++m) o << moff[m] << (m < 15 ? ", " : "");
+        o << "};\n    dense16_mfma(c0, mats + " << sd.dense << ", moff, " << sd.sbits[0] << ", " << sd.sbits[1]
+          << ", " << sd.sbits[2] << ", " << sd.sbits[3] << ", " << NG << ", "
+          << (sd.scale >= 0 ? "mats + " + std::to_string(sd.scale) : std::string("nullptr")) << ");\n  }\n";
+        o << "  __syncthreads();\n";
+        return;
+    }
     o << "  {\n";
     if (loop) o << "  for (int gb = 0; gb < " << NG << "; gb += kTPB) {\n";
     o << "    const int gi = " << (loop ? "gb + " : "") << "(int)threadIdx.x;\n";

@@ -316,7 +316,7 @@ void to_dense(Ctx &c, StepBuilder &sb)
 }  // namespace
 
 bool plan_passes(int n, const PlanOp *ops, int nops, const void *mats, int elem_bytes, int K, int cmin,
-                 int max_probes, Plan &out, std::string &err)
+                 int max_probes, Plan &out, std::string &err, bool fuse)
 {
     out = Plan();
     if (n < kG || n > 63) {
@@ -392,6 +392,8 @@ bool plan_passes(int n, const PlanOp *ops, int nops, const void *mats, int elem_
         std::vector<int> srem = chosen;
         while (!srem.empty()) {
             StepBuilder sb;
+            // fused plans: every step is one 16x16 product of all its ops
+            if (fuse && !any_probe) to_dense(ctx, sb);
             uint64_t sblk = 0;
             std::vector<int> sleft;
             for (int idx : srem) {

@@ -58,8 +58,10 @@ struct Plan {
 // as complex elements of `elem_bytes` (8: complex64, 16: complex128).  K is
 // clamped to n; cmin to K-2.  max_probes bounds the probe slots of one pass.
 // Returns false and sets err on invalid input.
+// fuse: every register step becomes one dense 16x16 matrix, the product of
+// all its ops (plans without probes only).
 bool plan_passes(int n, const PlanOp *ops, int nops, const void *mats, int elem_bytes, int K,
-                 int cmin, int max_probes, Plan &out, std::string &err);
+                 int cmin, int max_probes, Plan &out, std::string &err, bool fuse = false);
 
 // Pauli chunk passes: each term's X support must lie inside its pass's chunk.
 struct PauliPass {

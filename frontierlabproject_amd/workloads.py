@@ -20,23 +20,45 @@ from .circuit import QuantumCircuit
 RANDOM_GATESET = ("h", "x", "y", "z", "s", "t", "rx", "ry", "rz")
 
 
-def random_circuit(n, depth, seed, measure_marker=False):
+def random_gates(n, depth, seed):
+    """Gate stream of the seeded random circuit: (name, qubits, angle or None).
+    Per layer a gate from RANDOM_GATESET on every qubit (angles ~ U[0, 2 pi)),
+    then CX(control = perm[2i], target = perm[2i+1]) with p = 0.5 on the pairs
+    of a random permutation."""
     rng = np.random.Generator(np.random.PCG64(seed))
-    qc = QuantumCircuit(n)
     for _ in range(depth):
         for q in range(n):
             name = RANDOM_GATESET[int(rng.integers(0, len(RANDOM_GATESET)))]
             if name in ("rx", "ry", "rz"):
-                getattr(qc, name)(float(rng.uniform(0.0, 2 * math.pi)), q)
+                yield name, (q,), float(rng.uniform(0.0, 2 * math.pi))
             else:
-                getattr(qc, name)(q)
+                yield name, (q,), None
         perm = rng.permutation(n)
         for i in range(n // 2):
             if rng.random() < 0.5:
-                qc.cx(int(perm[2 * i]), int(perm[2 * i + 1]))
+                yield "cx", (int(perm[2 * i]), int(perm[2 * i + 1])), None
+
+
+def random_circuit(n, depth, seed, measure_marker=False):
+    qc = QuantumCircuit(n)
+    for name, qs, th in random_gates(n, depth, seed):
+        if th is None:
+            getattr(qc, name)(*qs)
+        else:
+            getattr(qc, name)(th, *qs)
     if measure_marker:
         qc.save_statevector()
     return qc
+
+
+def batch_spec(count=4096, seed=1004, nmin=12, nmax=22, dmin=10, dmax=40):
+    """The batched-circuits workload: (num_qubits, depth, circuit seed) per
+    circuit, qubit counts uniform in [nmin, nmax], depths uniform in
+    [dmin, dmax], circuit seeds seed * 100000 + index."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    ns = rng.integers(nmin, nmax + 1, size=count)
+    ds = rng.integers(dmin, dmax + 1, size=count)
+    return [(int(ns[i]), int(ds[i]), seed * 100000 + i) for i in range(count)]
 
 
 def qft_circuit(n, swaps=True):

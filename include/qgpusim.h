@@ -165,7 +165,16 @@ int qsim_set_jit(qsim_state *st, int enable);
  * kernel for gfx950.  Returns the number of passes, or -1 with the compiler
  * log. */
 int qsim_jit_compile_check(int num_qubits, int dtype, const qsim_op *ops, int nops,
-                           const void *mats, uint64_t mat_count, int K, int cmin);
+                           const void *mats, uint64_t mat_count, int K, int cmin, int fuse);
+
+/* No reference counterpart.  Fusion mode of forward chunk passes:
+ * 0 (default): slot programs (each gate by its own cheap arithmetic);
+ * 1: every register step is fused into one 4-qubit (16x16) unitary, the
+ * complex128 product of its gates rounded once, applied to the chunk as a
+ * complex GEMM on the FP32 matrix cores (three real 16x16x4 MFMA products
+ * per 16 columns, FP32 accumulation) when the pass kernel is specialised and
+ * the state is complex64, else by the vector units. */
+int qsim_set_fusion(qsim_state *st, int mode);
 
 /* Per-kernel-class timing with HIP events recorded on the state's stream around
  * each launch (not inside captured graphs).  Classes: 0 chunk pass,
@@ -189,6 +198,45 @@ int qsim_get_counters(const qsim_state *st, int64_t *out, int n);
 int qsim_plan_describe(int num_qubits, int dtype, const qsim_op *ops, int nops, const void *mats,
                        uint64_t mat_count, int K, int cmin, int32_t *desc, int desc_cap,
                        int32_t *order);
+
+/* ---- Batched circuits (no reference counterpart: the reference runs one
+ * circuit per call).  Many independent circuits run in one persistent
+ * launch: each workgroup takes circuits from a queue, largest first, and
+ * for each one runs its forward op list from |0...0>, reduces <Z_mask> in
+ * FP64, sets lam = Z_mask psi and runs its reverse list (the adjoint sweep,
+ * as for qsim_adjoint), writing every probe.  States of at most 12 qubits
+ * stay in LDS; larger ones stream through a per-workgroup HBM slot.
+ * complex64 only. */
+typedef struct qsim_batch qsim_batch;
+
+int qsim_batch_create(qsim_batch **out);
+int qsim_batch_destroy(qsim_batch *b);
+
+/* Chunk bits for streamed circuits by qubit count (K_by_n[n], n < nk, each in
+ * [6, 12]) and workgroups per CU (0: the occupancy maximum).  Applies to the
+ * next qsim_batch_load. */
+int qsim_batch_set_tuning(qsim_batch *b, const int32_t *K_by_n, int nk, int blocks_per_cu);
+
+/* Circuit i has num_qubits[i] (6..34) qubits, observable Z on zmask[i],
+ * forward ops fwd_ops[fwd_off[i] .. fwd_off[i+1]), reverse ops (QSIM_OP_PROBE
+ * allowed, q1 = probe slot < nprobes[i]) rev_ops[rev_off[i] .. rev_off[i+1]);
+ * mat_offset indexes the shared complex64 arena `mats` (mat_count elements).
+ * Plans every circuit and uploads one descriptor image. */
+int qsim_batch_load(qsim_batch *b, int ncirc, const int32_t *num_qubits, const uint64_t *zmask,
+                    const qsim_op *fwd_ops, const int64_t *fwd_off, const qsim_op *rev_ops,
+                    const int64_t *rev_off, const void *mats, uint64_t mat_count,
+                    const int32_t *nprobes);
+
+/* Runs the loaded batch (use_graph: replay a hipGraph captured on first use)
+ * and synchronises.  Outputs (each may be NULL): expect_out[ncirc],
+ * probes_out[sum nprobes] (circuit order), exec_ms_out[ncirc] (each
+ * circuit's time on its workgroup), *total_ms_out (the launch, HIP events). */
+int qsim_batch_run(qsim_batch *b, int use_graph, double *expect_out, double *probes_out,
+                   double *exec_ms_out, double *total_ms_out);
+
+/* [0] circuits, [1] probes, [2] passes, [3] workgroups, [4] HBM slot bytes.
+ * Returns the number written. */
+int qsim_batch_info(const qsim_batch *b, int64_t *out, int n);
 
 #ifdef __cplusplus
 }

@@ -288,6 +288,8 @@ def test_specialised_pass_kernels_compile(dtype, K, cmin):
     npass = hip.jit_compile_check(n, arr, len(ops), arena, K, cmin)
     passes, _ = hip.plan_describe(n, arr, len(ops), arena, K, cmin)
     assert npass == len(passes) > 0
+    # fused steps (matrix-core GEMMs for complex64) compile too
+    assert hip.jit_compile_check(n, arr, len(ops), arena, K, cmin, fuse=True) > 0
 
 
 @pytest.mark.parametrize("use_float32", [True, False])
@@ -303,3 +305,59 @@ def test_specialised_adjoint_kernels_compile(use_float32):
     arena = buf.arena()
     K, cmin = (12, 4) if use_float32 else (11, 3)
     assert hip.jit_compile_check(n, buf.arr, buf.n, arena, K, cmin) > 0
+
+
+def test_batch_pack_matches_gate_matrices():
+    """Packed batch op lists: forward matrices are the gates', the reverse list
+    is the inverses in reverse order with each rotation preceded by a probe of
+    its generator in the rotation's forward slot."""
+    from frontierlabproject_amd import batch as B
+    spec = [(7, 4, 11), (13, 3, 12), (9, 5, 13)]
+    circs = B.random_batch(spec)
+    p = B.pack(circs)
+    mats = p["mats"].astype(np.complex128)
+    gens = {"rx": G.x(), "ry": G.y(), "rz": G.z()}
+    for i, (n, gates) in enumerate(circs):
+        fwd = p["fwd"][p["fwd_off"][i]:p["fwd_off"][i + 1]]
+        rev = p["rev"][p["rev_off"][i]:p["rev_off"][i + 1]]
+        assert len(fwd) == len(gates)
+        rots = []
+        for o, (name, qs, th) in zip(fwd, gates):
+            U = G.matrix(name, [th] if th is not None else [])
+            k = U.shape[0] ** 2
+            got = mats[o["mat_offset"]:o["mat_offset"] + k].reshape(U.shape)
+            assert np.allclose(got, U, atol=1e-6), name
+            assert o["q0"] == qs[0] and (o["q1"] == qs[1] if len(qs) == 2 else o["q1"] == -1)
+            if th is not None:
+                rots.append(name)
+        assert p["nprobe"][i] == len(rots)
+        j = 0
+        for name, qs, th in reversed(gates):
+            U = G.matrix(name, [th] if th is not None else [])
+            if th is not None:
+                pr = rev[j]
+                assert pr["kind"] == 3 and pr["q0"] == qs[0] and pr["q1"] == len(rots) - 1
+                rots.pop()
+                g = mats[pr["mat_offset"]:pr["mat_offset"] + 4].reshape(2, 2)
+                assert np.allclose(g, gens[name])
+                j += 1
+            o = rev[j]
+            k = U.shape[0] ** 2
+            got = mats[o["mat_offset"]:o["mat_offset"] + k].reshape(U.shape)
+            assert np.allclose(got, U.conj().T, atol=1e-6)
+            j += 1
+        assert j == len(rev)
+
+
+def test_batch_spec_and_generator_are_reproducible():
+    spec = W.batch_spec()
+    assert len(spec) == 4096
+    assert spec == W.batch_spec()
+    ns = np.array([s[0] for s in spec])
+    ds = np.array([s[1] for s in spec])
+    assert ns.min() == 12 and ns.max() == 22 and ds.min() == 10 and ds.max() == 40
+    # the gate stream is the random circuit's
+    n, d, seed = spec[5]
+    qc = W.random_circuit(n, d, seed)
+    stream = list(W.random_gates(n, d, seed))
+    assert [ci.operation.name for ci in qc.data] == [g[0] for g in stream]
