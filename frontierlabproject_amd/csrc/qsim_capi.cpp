@@ -296,8 +296,9 @@ int enqueue_passes(qsim_state *st, const Plan &plan, const DescImage &img, char 
             const void *mats = dm;
             void *args1[] = {&psi, &mats};
             void *args2[] = {&psi, &lamp, &part, &mats};
-            HIP_TRY(hipModuleLaunchKernel((*jit)[p], (unsigned)nblocks, 1, 1, kTPB, 1, 1, 0, st->stream,
-                                          lam ? args2 : args1, nullptr));
+            const int nv = lam ? 2 : 1;
+            HIP_TRY(hipModuleLaunchKernel((*jit)[p], (unsigned)nblocks, 1, 1, (unsigned)jit_block_threads(hp.K, nv),
+                                          1, 1, 0, st->stream, lam ? args2 : args1, nullptr));
             st->counters[5]++;
         } else {
             HIP_TRY(launch_tile_pass<R>(static_cast<cx<R> *>(st->psi), static_cast<cx<R> *>(lam), st->n, hp, dp + p,

@@ -54,7 +54,10 @@ typedef float v2f __attribute__((ext_vector_type(2)));
 template <typename R> struct RawAmp { typedef v2f type; };
 template <> struct RawAmp<double> { typedef v4f type; };
 
-constexpr int kBatch = 8;  // global accesses in flight per thread in the copy loops
+#ifndef QSIM_KBATCH
+#define QSIM_KBATCH 8
+#endif
+constexpr int kBatch = QSIM_KBATCH;  // global accesses in flight per thread in the copy loops
 
 // ----------------------------------------------------------------------------
 // Register-step building blocks: 16 amplitudes per thread over 4 register bits;
@@ -344,36 +347,36 @@ __device__ __forceinline__ void gstore(V *p, V v)
         *p = v;
 }
 
-template <typename R, bool LOAD, bool NT = false>
+template <typename R, bool LOAD, bool NT = false, int TPB = kTPB>
 __device__ __forceinline__ void chunk_copy(cx<R> *__restrict__ g, cx<R> *__restrict__ lds, uint64_t base,
                                            const uint64_t *__restrict__ rowoff, int E, int cf)
 {
     const int lowmask = (1 << cf) - 1;
     if (sizeof(cx<R>) == 8 && cf >= 1) {
         const int nv = E >> 1;
-        for (int v0 = 0; v0 < nv; v0 += kTPB * kBatch) {
+        for (int v0 = 0; v0 < nv; v0 += TPB * kBatch) {
             v4f buf[kBatch];
             if (LOAD) {
 #pragma unroll
                 for (int j = 0; j < kBatch; ++j) {
-                    const int v = v0 + j * kTPB + threadIdx.x;
+                    const int v = v0 + j * TPB + threadIdx.x;
                     const int e = v << 1;
                     if (v < nv) buf[j] = gload<NT>(reinterpret_cast<const v4f *>(g + base + rowoff[e >> cf] + (e & lowmask)));
                 }
 #pragma unroll
                 for (int j = 0; j < kBatch; ++j) {
-                    const int v = v0 + j * kTPB + threadIdx.x;
+                    const int v = v0 + j * TPB + threadIdx.x;
                     if (v < nv) *reinterpret_cast<v4f *>(lds + swz(v << 1)) = buf[j];
                 }
             } else {
 #pragma unroll
                 for (int j = 0; j < kBatch; ++j) {
-                    const int v = v0 + j * kTPB + threadIdx.x;
+                    const int v = v0 + j * TPB + threadIdx.x;
                     if (v < nv) buf[j] = *reinterpret_cast<const v4f *>(lds + swz(v << 1));
                 }
 #pragma unroll
                 for (int j = 0; j < kBatch; ++j) {
-                    const int v = v0 + j * kTPB + threadIdx.x;
+                    const int v = v0 + j * TPB + threadIdx.x;
                     const int e = v << 1;
                     if (v < nv) gstore<NT>(reinterpret_cast<v4f *>(g + base + rowoff[e >> cf] + (e & lowmask)), buf[j]);
                 }
@@ -384,28 +387,28 @@ __device__ __forceinline__ void chunk_copy(cx<R> *__restrict__ g, cx<R> *__restr
         const raw *gr = reinterpret_cast<const raw *>(g);
         raw *gw = reinterpret_cast<raw *>(g);
         raw *lr = reinterpret_cast<raw *>(lds);
-        for (int e0 = 0; e0 < E; e0 += kTPB * kBatch) {
+        for (int e0 = 0; e0 < E; e0 += TPB * kBatch) {
             raw buf[kBatch];
             if (LOAD) {
 #pragma unroll
                 for (int j = 0; j < kBatch; ++j) {
-                    const int e = e0 + j * kTPB + threadIdx.x;
+                    const int e = e0 + j * TPB + threadIdx.x;
                     if (e < E) buf[j] = gload<NT>(gr + base + rowoff[e >> cf] + (e & lowmask));
                 }
 #pragma unroll
                 for (int j = 0; j < kBatch; ++j) {
-                    const int e = e0 + j * kTPB + threadIdx.x;
+                    const int e = e0 + j * TPB + threadIdx.x;
                     if (e < E) lr[swz(e)] = buf[j];
                 }
             } else {
 #pragma unroll
                 for (int j = 0; j < kBatch; ++j) {
-                    const int e = e0 + j * kTPB + threadIdx.x;
+                    const int e = e0 + j * TPB + threadIdx.x;
                     if (e < E) buf[j] = lr[swz(e)];
                 }
 #pragma unroll
                 for (int j = 0; j < kBatch; ++j) {
-                    const int e = e0 + j * kTPB + threadIdx.x;
+                    const int e = e0 + j * TPB + threadIdx.x;
                     if (e < E) gstore<NT>(gw + base + rowoff[e >> cf] + (e & lowmask), buf[j]);
                 }
             }
@@ -532,6 +535,7 @@ __device__ __forceinline__ void run_steps(cx<R> *c0, cx<R> *c1, const PassDev *_
 // read and written by one lane set of one wave, in program order.
 // moff[m]: LDS slot offset of register index m; s0..s3: the step's register
 // bits ascending; ng: register groups in the chunk; sc: optional factor.
+template <int TPB = kTPB>
 __device__ __forceinline__ void dense16_mfma(cx<float> *c0, const cx<float> *__restrict__ M, const int *moff,
                                              int s0, int s1, int s2, int s3, int ng, const cx<float> *sc)
 {
@@ -546,7 +550,7 @@ __device__ __forceinline__ void dense16_mfma(cx<float> *c0, const cx<float> *__r
         as[kb] = m.re + m.im;
     }
     const cx<float> f = sc ? *sc : cx<float>{1.f, 0.f};
-    for (int gb = wave * 64; gb < ng; gb += kTPB) {  // wave-uniform trip count
+    for (int gb = wave * 64; gb < ng; gb += TPB) {  // wave-uniform trip count
 #pragma unroll
         for (int nb = 0; nb < 4; ++nb) {
             const int g = gb + 16 * nb + j;

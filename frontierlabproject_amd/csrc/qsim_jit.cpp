@@ -44,25 +44,25 @@ bool skip_pass(const Plan &plan, size_t p, int nv) { return nv == 1 && plan.pass
 constexpr int kPairF[6] = {0, 2, 0, 1, 0, 1};
 constexpr int kPairS[6] = {1, 3, 2, 3, 3, 2};
 
-void emit_step(std::ostringstream &o, const StepDev &sd, int E, const char *T, int nv)
+void emit_step(std::ostringstream &o, const StepDev &sd, int E, const char *T, int nv, int tpb)
 {
     const int NG = E >> kG;
     int moff[16];
     for (int m = 0; m < 16; ++m)
         moff[m] = swz_host(((m & 1) << sd.bits[0]) | (((m >> 1) & 1) << sd.bits[1]) |
                            (((m >> 2) & 1) << sd.bits[2]) | (((m >> 3) & 1) << sd.bits[3]));
-    const bool loop = NG > kTPB, guard = NG < kTPB;
+    const bool loop = NG > tpb, guard = NG < tpb;
     if (sd.dense >= 0 && nv == 1 && std::string(T) == "float") {  // matrix cores
         o << "  { const int moff[16] = {";
         for (int m = 0; m < 16; ++m) o << moff[m] << (m < 15 ? ", " : "");
-        o << "};\n    dense16_mfma(c0, mats + " << sd.dense << ", moff, " << sd.sbits[0] << ", " << sd.sbits[1]
+        o << "};\n    dense16_mfma<" << tpb << ">(c0, mats + " << sd.dense << ", moff, " << sd.sbits[0] << ", " << sd.sbits[1]
           << ", " << sd.sbits[2] << ", " << sd.sbits[3] << ", " << NG << ", "
           << (sd.scale >= 0 ? "mats + " + std::to_string(sd.scale) : std::string("nullptr")) << ");\n  }\n";
         o << "  __syncthreads();\n";
         return;
     }
     o << "  {\n";
-    if (loop) o << "  for (int gb = 0; gb < " << NG << "; gb += kTPB) {\n";
+    if (loop) o << "  for (int gb = 0; gb < " << NG << "; gb += " << tpb << ") {\n";
     o << "    const int gi = " << (loop ? "gb + " : "") << "(int)threadIdx.x;\n";
     if (guard)
         o << "    const bool act = gi < " << NG << ";\n    const uint32_t g32 = (uint32_t)(act ? gi : " << NG - 1
@@ -151,9 +151,22 @@ bool compile_unit(const std::string &src, const std::string &arch, std::vector<c
         return false;
     }
     const std::string af = arch_flag(arch);
-    const char *opts[] = {af.c_str(), "-O3", "-std=c++17", "-ffp-contract=fast",
-                          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"};
-    r = hiprtcCompileProgram(prog, (int)(sizeof(opts) / sizeof(opts[0])), opts);
+    std::vector<std::string> extra;  // QSIM_JIT_FLAGS: further compiler flags, space-separated
+    if (const char *e = std::getenv("QSIM_JIT_FLAGS")) {
+        std::string f = e, tok;
+        for (char ch : f + " ") {
+            if (ch == ' ') {
+                if (!tok.empty()) extra.push_back(tok);
+                tok.clear();
+            } else {
+                tok += ch;
+            }
+        }
+    }
+    std::vector<const char *> opts = {af.c_str(), "-O3", "-std=c++17", "-ffp-contract=fast",
+                                      "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"};
+    for (auto &x : extra) opts.push_back(x.c_str());
+    r = hiprtcCompileProgram(prog, (int)opts.size(), opts.data());
     if (r != HIPRTC_SUCCESS) {
         size_t n = 0;
         hiprtcGetProgramLogSize(prog, &n);
@@ -221,9 +234,10 @@ std::string jit_pass_source(const Plan &plan, size_t p, bool dbl, int nv, const 
     const char *T = dbl ? "double" : "float";
     const int K = pd.K, CF = pd.cfast, E = 1 << K, NROW = 1 << (K - CF);
     const int NP = nv == 2 ? std::max(1, pd.nprobe) : 0;
+    const int TPB = jit_block_threads(K, nv);
+    const char *occ = dbl || nv == 2 ? "" : " __attribute__((amdgpu_waves_per_eu(4)))";
     std::ostringstream o;
-    o << "extern \"C\" __global__ __launch_bounds__(kTPB)"
-      << (dbl || nv == 2 ? "" : " __attribute__((amdgpu_waves_per_eu(4)))") << "\nvoid " << name << "(cx<" << T
+    o << "extern \"C\" __global__ __launch_bounds__(" << TPB << ")" << occ << "\nvoid " << name << "(cx<" << T
       << "> *__restrict__ psi, ";
     if (nv == 2) o << "cx<" << T << "> *__restrict__ lam, double *__restrict__ probe_partial, ";
     o << "const cx<" << T << "> *__restrict__ mats)\n{\n";
@@ -234,24 +248,26 @@ std::string jit_pass_source(const Plan &plan, size_t p, bool dbl, int nv, const 
         o << "  for (int t = threadIdx.x; t < " << NP << " * kWaves; t += kTPB) pacc[t] = 0.0;\n";
     }
     o << "  __shared__ uint64_t rowoff[" << NROW << "];\n";
-    o << "  for (int t = threadIdx.x; t < " << NROW << "; t += kTPB) {\n    uint64_t r = 0;\n";
+    o << "  for (int t = threadIdx.x; t < " << NROW << "; t += " << TPB << ") {\n    uint64_t r = 0;\n";
     for (int k = CF; k < K; ++k)
         o << "    r |= (uint64_t)((t >> " << k - CF << ") & 1) << " << pd.pos[k] << ";\n";
     o << "    rowoff[t] = r;\n  }\n";
-    o << "  const uint64_t c = xcd_slab(blockIdx.x, " << pd.ncomp << ");\n";
-    o << "  uint64_t base = 0;\n";
-    for (int m = 0; m < pd.ncomp;) {  // runs of consecutive positions move as one field
+    // chunk index -> base address: runs of consecutive positions move as one field
+    std::ostringstream dep;
+    for (int m = 0; m < pd.ncomp;) {
         int len = 1;
         while (m + len < pd.ncomp && pd.comp[m + len] == pd.comp[m] + len) ++len;
-        o << "  base |= ((c >> " << m << ") & " << ((1ull << len) - 1) << "ull) << " << pd.comp[m] << ";\n";
+        dep << " | (((c >> " << m << ") & " << ((1ull << len) - 1) << "ull) << " << pd.comp[m] << ")";
         m += len;
     }
+    o << "  const uint64_t c = xcd_slab(blockIdx.x, " << pd.ncomp << ");\n";
+    o << "  const uint64_t base = 0ull" << dep.str() << ";\n";
     o << "  __syncthreads();\n";
-    const std::string ntf = ", true";  // streamed once: non-temporal global accesses
+    const std::string ntf = ", true, " + std::to_string(TPB);  // non-temporal (streamed once), block size
     o << "  chunk_copy<" << T << ", true" << ntf << ">(psi, c0, base, rowoff, " << E << ", " << CF << ");\n";
     if (nv == 2) o << "  chunk_copy<" << T << ", true" << ntf << ">(lam, c1, base, rowoff, " << E << ", " << CF << ");\n";
     o << "  __syncthreads();\n";
-    for (int s = 0; s < pd.nsteps; ++s) emit_step(o, plan.steps[pd.step_off + s], E, T, nv);
+    for (int s = 0; s < pd.nsteps; ++s) emit_step(o, plan.steps[pd.step_off + s], E, T, nv, TPB);
     o << "  chunk_copy<" << T << ", false" << ntf << ">(psi, c0, base, rowoff, " << E << ", " << CF << ");\n";
     if (nv == 2) {
         o << "  chunk_copy<" << T << ", false" << ntf << ">(lam, c1, base, rowoff, " << E << ", " << CF << ");\n";
@@ -279,6 +295,7 @@ bool jit_functions(const Plan &plan, bool dbl, int nv, int device, const std::st
         for (size_t p = 0; p < np; ++p) {
             if (skip_pass(plan, p, nv)) continue;
             keys[p] = jit_pass_source(plan, p, dbl, nv, "@");
+            if (const char *e = std::getenv("QSIM_JIT_FLAGS")) keys[p] = std::string("// ") + e + "\n" + keys[p];
             auto it = g_fns.find({device, keys[p]});
             if (it != g_fns.end()) {
                 out[p] = it->second;
@@ -332,5 +349,12 @@ bool jit_compile_only(const Plan &plan, bool dbl, int nv, const std::string &arc
 }
 
 long jit_compiled_count() { return g_compiled.load(); }
+
+int jit_block_threads(int K, int nv)
+{
+    // one 16-amplitude register group per thread per step: 2^K / 16 threads,
+    // at least 256; two-vector passes keep 256 (their probe partials are per wave)
+    return (nv == 1 && K >= 13) ? 512 : kTPB;
+}
 
 }  // namespace qsim

@@ -38,4 +38,7 @@ bool jit_compile_only(const Plan &plan, bool dbl, int nv, const std::string &arc
 // Kernels compiled so far in this process (all devices).
 long jit_compiled_count();
 
+// Threads per block of the specialised kernel of a pass with K chunk bits.
+int jit_block_threads(int K, int nv);
+
 }  // namespace qsim

@@ -262,8 +262,8 @@ hipError_t launch_diag1q(cx<R> *psi, int n, int q, cx<R> p0, cx<R> p1, hipStream
 // contiguous output amplitudes.  Tile slot r (input order) is stored at
 // r ^ ((r >> a) & (2^a - 1)) so that the output-order reads spread over banks.
 // ----------------------------------------------------------------------------
-template <typename R, bool NT>
-__global__ __launch_bounds__(kTPB) void k_permute(const cx<R> *__restrict__ in, cx<R> *__restrict__ out, PermArgs P)
+template <typename R, bool NT, int TPB>
+__global__ __launch_bounds__(TPB) void k_permute(const cx<R> *__restrict__ in, cx<R> *__restrict__ out, PermArgs P)
 {
     extern __shared__ __align__(16) unsigned char smem[];
     const int s = P.s, a = P.a, b = P.b;
@@ -274,13 +274,13 @@ __global__ __launch_bounds__(kTPB) void k_permute(const cx<R> *__restrict__ in, 
     uint32_t *w2r_lo = reinterpret_cast<uint32_t *>(row_out + (1 << (s - b)));
     uint32_t *w2r_hi = w2r_lo + (1 << b);
 
-    for (int t = threadIdx.x; t < (1 << (s - a)); t += kTPB) {
+    for (int t = threadIdx.x; t < (1 << (s - a)); t += TPB) {
         uint64_t o = 0;
         for (int k = 0; k < s - a; ++k)
             if ((t >> k) & 1) o |= 1ull << P.sin_hi[k];
         row_in[t] = o;
     }
-    for (int t = threadIdx.x; t < (1 << (s - b)); t += kTPB) {
+    for (int t = threadIdx.x; t < (1 << (s - b)); t += TPB) {
         uint64_t o = 0;
         uint32_t r = 0;
         for (int k = 0; k < s - b; ++k)
@@ -291,7 +291,7 @@ __global__ __launch_bounds__(kTPB) void k_permute(const cx<R> *__restrict__ in, 
         row_out[t] = o;
         w2r_hi[t] = r;
     }
-    for (int t = threadIdx.x; t < (1 << b); t += kTPB) {
+    for (int t = threadIdx.x; t < (1 << b); t += TPB) {
         uint32_t r = 0;
         for (int k = 0; k < b; ++k)
             if ((t >> k) & 1) r |= 1u << P.w2r[k];
@@ -312,26 +312,26 @@ __global__ __launch_bounds__(kTPB) void k_permute(const cx<R> *__restrict__ in, 
     auto slot = [=](uint32_t r) { return r ^ (((r >> a) << 1) & amask); };
     if (sizeof(cx<R>) == 8 && a >= 1 && b >= 1) {
         const int nv = S >> 1;
-        for (int v0 = 0; v0 < nv; v0 += kTPB * kBatch) {
+        for (int v0 = 0; v0 < nv; v0 += TPB * kBatch) {
             v4f buf[kBatch];
 #pragma unroll
             for (int j = 0; j < kBatch; ++j) {
-                const int v = v0 + j * kTPB + threadIdx.x;
+                const int v = v0 + j * TPB + threadIdx.x;
                 const int r = v << 1;
                 if (v < nv) buf[j] = gload<NT>(reinterpret_cast<const v4f *>(in + base_in + row_in[r >> a] + (r & amask)));
             }
 #pragma unroll
             for (int j = 0; j < kBatch; ++j) {
-                const int v = v0 + j * kTPB + threadIdx.x;
+                const int v = v0 + j * TPB + threadIdx.x;
                 if (v < nv) *reinterpret_cast<v4f *>(tile + slot(v << 1)) = buf[j];
             }
         }
         __syncthreads();
-        for (int v0 = 0; v0 < nv; v0 += kTPB * kBatch) {
+        for (int v0 = 0; v0 < nv; v0 += TPB * kBatch) {
             v4f buf[kBatch];
 #pragma unroll
             for (int j = 0; j < kBatch; ++j) {
-                const int v = v0 + j * kTPB + threadIdx.x;
+                const int v = v0 + j * TPB + threadIdx.x;
                 const int w = v << 1;
                 if (v < nv) {
                     const uint32_t r0 = w2r_lo[w & bmask] | w2r_hi[w >> b];
@@ -342,7 +342,7 @@ __global__ __launch_bounds__(kTPB) void k_permute(const cx<R> *__restrict__ in, 
             }
 #pragma unroll
             for (int j = 0; j < kBatch; ++j) {
-                const int v = v0 + j * kTPB + threadIdx.x;
+                const int v = v0 + j * TPB + threadIdx.x;
                 const int w = v << 1;
                 if (v < nv) gstore<NT>(reinterpret_cast<v4f *>(out + base_out + row_out[w >> b] + (w & bmask)), buf[j]);
             }
@@ -352,30 +352,30 @@ __global__ __launch_bounds__(kTPB) void k_permute(const cx<R> *__restrict__ in, 
         const raw *ir = reinterpret_cast<const raw *>(in);
         raw *ow = reinterpret_cast<raw *>(out);
         raw *tr = reinterpret_cast<raw *>(tile);
-        for (int r0 = 0; r0 < S; r0 += kTPB * kBatch) {
+        for (int r0 = 0; r0 < S; r0 += TPB * kBatch) {
             raw buf[kBatch];
 #pragma unroll
             for (int j = 0; j < kBatch; ++j) {
-                const int r = r0 + j * kTPB + threadIdx.x;
+                const int r = r0 + j * TPB + threadIdx.x;
                 if (r < S) buf[j] = gload<NT>(ir + base_in + row_in[r >> a] + (r & amask));
             }
 #pragma unroll
             for (int j = 0; j < kBatch; ++j) {
-                const int r = r0 + j * kTPB + threadIdx.x;
+                const int r = r0 + j * TPB + threadIdx.x;
                 if (r < S) tr[slot(r)] = buf[j];
             }
         }
         __syncthreads();
-        for (int w0 = 0; w0 < S; w0 += kTPB * kBatch) {
+        for (int w0 = 0; w0 < S; w0 += TPB * kBatch) {
             raw buf[kBatch];
 #pragma unroll
             for (int j = 0; j < kBatch; ++j) {
-                const int w = w0 + j * kTPB + threadIdx.x;
+                const int w = w0 + j * TPB + threadIdx.x;
                 if (w < S) buf[j] = tr[slot(w2r_lo[w & bmask] | w2r_hi[w >> b])];
             }
 #pragma unroll
             for (int j = 0; j < kBatch; ++j) {
-                const int w = w0 + j * kTPB + threadIdx.x;
+                const int w = w0 + j * TPB + threadIdx.x;
                 if (w < S) gstore<NT>(ow + base_out + row_out[w >> b] + (w & bmask), buf[j]);
             }
         }
@@ -394,7 +394,7 @@ hipError_t launch_permute(const cx<R> *in, cx<R> *out, const PermArgs &P, hipStr
 {
     const size_t sm = perm_smem<R>(P);
     const uint64_t blocks = 1ull << (P.n - P.s);
-    hipLaunchKernelGGL((k_permute<R, true>), dim3((unsigned)blocks), dim3(kTPB), sm, s, in, out, P);
+    hipLaunchKernelGGL((k_permute<R, true, kTPB>), dim3((unsigned)blocks), dim3(kTPB), sm, s, in, out, P);
     return hipGetLastError();
 }
 
@@ -661,7 +661,8 @@ hipError_t prepare_kernels()
     const void *fns[] = {
         reinterpret_cast<const void *>(k_tile_pass<float, 1>), reinterpret_cast<const void *>(k_tile_pass<float, 2>),
         reinterpret_cast<const void *>(k_tile_pass<double, 1>), reinterpret_cast<const void *>(k_tile_pass<double, 2>),
-        reinterpret_cast<const void *>(k_permute<float, true>), reinterpret_cast<const void *>(k_permute<double, true>),
+        reinterpret_cast<const void *>(k_permute<float, true, kTPB>),
+        reinterpret_cast<const void *>(k_permute<double, true, kTPB>),
         reinterpret_cast<const void *>(k_pauli_chunk<float>), reinterpret_cast<const void *>(k_pauli_chunk<double>),
     };
     for (const void *f : fns) hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
