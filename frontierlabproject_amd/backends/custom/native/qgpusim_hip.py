@@ -83,7 +83,7 @@ _SIGNATURES = {
     "qsim_batch_load": ([_P, _I, _P, _P, _P, _P, _P, _P, _P, _U64, _P], _I),
     "qsim_batch_run": ([_P, _I, _P, _P, _P, _P], _I),
     "qsim_batch_info": ([_P, _P, _I], _I),
-    "qsim_jit_compile_check": ([_I, _I, _P, _I, _P, _U64, _I, _I, _I], _I),
+    "qsim_jit_compile_check": ([_I, _I, _P, _I, _P, _U64, _I, _I, _I, _P], _I),
     "qsim_set_fusion": ([_P, _I], _I),
 }
 
@@ -355,10 +355,10 @@ class _StatevectorBase:
         return {names[i]: (float(ms[i]), int(cnt[i])) for i in range(m)}
 
     def counters(self):
-        buf = (ctypes.c_int64 * 6)()
-        lib.qsim_get_counters(self._h, ctypes.addressof(buf), 6)
+        buf = (ctypes.c_int64 * 7)()
+        lib.qsim_get_counters(self._h, ctypes.addressof(buf), 7)
         keys = ("kernel_launches", "graph_launches", "graph_captures", "chunk_passes", "permutations",
-                "specialized_passes")
+                "specialized_passes", "folded_permutations")
         return dict(zip(keys, list(buf)))
 
 
@@ -437,12 +437,15 @@ def validate_ops(ops, num_qubits):
     _check(lib.qsim_validate_ops(num_qubits, ctypes.addressof(arr), len(ops), arena.size))
 
 
-def jit_compile_check(num_qubits, arr, nops, arena, K, cmin, fuse=False):
-    """Host-only: plan packed ops and compile every specialised pass kernel for
-    gfx950 without loading it; returns the number of passes."""
+def jit_compile_check(num_qubits, arr, nops, arena, K, cmin, fuse=False, in_map=None):
+    """Host-only: plan packed ops (after a pending layout change in_map, bit p
+    -> in_map[p]) and compile every specialised pass kernel for gfx950 without
+    loading it; returns the number of passes."""
     code = C128 if arena.dtype == np.complex128 else C64
+    m = None if in_map is None else np.ascontiguousarray(in_map, dtype=np.int32)
     npass = lib.qsim_jit_compile_check(num_qubits, code, ctypes.addressof(arr), nops, arena.ctypes.data,
-                                       arena.size, K, cmin, 1 if fuse else 0)
+                                       arena.size, K, cmin, 1 if fuse else 0,
+                                       None if m is None else m.ctypes.data)
     if npass < 0:
         raise RuntimeError(lib.qsim_last_error().decode())
     return npass

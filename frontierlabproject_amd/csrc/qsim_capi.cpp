@@ -102,6 +102,7 @@ struct GraphEntry {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     void *desc = nullptr;
+    bool swaps = false;  // its first pass writes the other buffer (folded layout change)
 };
 
 size_t amp_bytes(int dtype) { return dtype == QSIM_C64 ? 8 : 16; }
@@ -126,8 +127,13 @@ struct qsim_state {
     double *d_probe = nullptr;     // probe partials
     size_t probe_cap = 0;          // doubles
     // kernel launches, graph launches, graph captures, chunk passes,
-    // permutations, chunk passes run by specialised kernels
-    int64_t counters[6] = {0, 0, 0, 0, 0, 0};
+    // permutations run, chunk passes run by specialised kernels, permutations
+    // folded into a chunk pass
+    int64_t counters[7] = {0, 0, 0, 0, 0, 0, 0};
+    // layout change requested by qsim_permute and not yet applied:
+    // perm_map[p] = new position of the current bit position p
+    bool perm_pending = false;
+    int32_t perm_map[64];
     bool jit = true;               // specialise forward chunk passes at run time
     bool fuse = false;             // every register step one 16x16 unitary (matrix cores)
     std::string arch;              // device gcnArchName (for run-time compiles)
@@ -183,6 +189,31 @@ struct ProfScope {
 int ensure_tmp(qsim_state *st)
 {
     if (!st->tmp) HIP_TRY(hipMalloc(&st->tmp, st->dim * amp_bytes(st->dtype)));
+    return 0;
+}
+
+int build_perm_args(int n, const int32_t *map, int smax, PermArgs &P);
+
+// Applies a pending layout change with the permutation kernel.
+int flush_perm(qsim_state *st)
+{
+    if (!st->perm_pending) return 0;
+    if (ensure_tmp(st)) return -1;
+    PermArgs P;
+    build_perm_args(st->n, st->perm_map, st->dtype == QSIM_C64 ? 12 : 11, P);
+    {
+        ProfScope ps(st, 1);
+        if (st->dtype == QSIM_C64)
+            HIP_TRY(launch_permute<float>(static_cast<cx<float> *>(st->psi), static_cast<cx<float> *>(st->tmp), P,
+                                          st->stream));
+        else
+            HIP_TRY(launch_permute<double>(static_cast<cx<double> *>(st->psi), static_cast<cx<double> *>(st->tmp), P,
+                                           st->stream));
+    }
+    std::swap(st->psi, st->tmp);
+    st->perm_pending = false;
+    st->counters[0]++;
+    st->counters[4]++;
     return 0;
 }
 
@@ -289,7 +320,18 @@ int enqueue_passes(qsim_state *st, const Plan &plan, const DescImage &img, char 
         const PassDev &hp = plan.passes[p];
         const int nblocks = 1 << (st->n - hp.K);
         ProfScope ps(st, 0);
-        if (jit && p < jit->size() && (*jit)[p]) {
+        if (p == 0 && !plan.in_map.empty()) {
+            // folded layout change: read psi in the old layout, write tmp in
+            // the new one; later passes work in place on the new buffer
+            void *in = st->psi, *out = st->tmp;
+            const void *mats = dm;
+            void *args[] = {&in, &out, &mats};
+            HIP_TRY(hipModuleLaunchKernel((*jit)[0], (unsigned)nblocks, 1, 1, (unsigned)jit_block_threads(hp.K, 1),
+                                          1, 1, 0, st->stream, args, nullptr));
+            std::swap(st->psi, st->tmp);
+            st->counters[5]++;
+            st->counters[6]++;
+        } else if (jit && p < jit->size() && (*jit)[p]) {
             void *psi = st->psi;
             void *lamp = lam;
             double *part = st->d_probe;
@@ -314,12 +356,16 @@ int enqueue_passes(qsim_state *st, const Plan &plan, const DescImage &img, char 
     return 0;
 }
 
-int plan_for(qsim_state *st, const qsim_op *ops, int nops, const void *mats, int max_probes, Plan &plan)
+int plan_for(qsim_state *st, const qsim_op *ops, int nops, const void *mats, int max_probes, Plan &plan,
+             const int32_t *in_map = nullptr)
 {
     std::vector<PlanOp> pops = to_plan_ops(ops, nops);
     std::string err;
-    if (!plan_passes(st->n, pops.data(), nops, mats, (int)amp_bytes(st->dtype), st->K, st->cmin, max_probes, plan,
-                     err, st->fuse && max_probes == 1))
+    // two-vector (probe) passes hold two chunks in LDS: at most 2^12 complex64
+    // amplitudes each
+    const int K = max_probes > 1 ? std::min(st->K, st->dtype == QSIM_C64 ? 12 : 11) : st->K;
+    if (!plan_passes(st->n, pops.data(), nops, mats, (int)amp_bytes(st->dtype), K, st->cmin, max_probes, plan,
+                     err, st->fuse && max_probes == 1, in_map))
         return fail(err);
     return 0;
 }
@@ -340,14 +386,36 @@ int jit_for(qsim_state *st, const Plan &plan, bool dbl, int nv, std::vector<hipF
     return 0;
 }
 
+// A pending layout change folds into the first chunk pass of the next op
+// list when that pass runs a specialised complex64 kernel.
+bool can_fold(const qsim_state *st) { return st->perm_pending && st->jit && st->dtype == QSIM_C64; }
+
+// Plans `ops`, folding a pending layout change into the first pass when it
+// can (else applies it first), and fetches the specialised kernels.
+template <typename R>
+int plan_and_compile(qsim_state *st, const qsim_op *ops, int nops, const void *mats, Plan &plan,
+                     std::vector<hipFunction_t> &fns)
+{
+    const bool fold = can_fold(st);
+    if (st->perm_pending && !fold && flush_perm(st)) return -1;
+    if (plan_for(st, ops, nops, mats, 1, plan, fold ? st->perm_map : nullptr)) return -1;
+    if (jit_for(st, plan, sizeof(R) == 8, 1, fns)) return -1;
+    if (fold) {
+        if (!jit_mapped_first(plan, sizeof(R) == 8, 1) || fns.empty() || !fns[0])
+            return fail("internal: folded layout change without its kernel");
+        if (ensure_tmp(st)) return -1;
+        st->perm_pending = false;
+    }
+    return 0;
+}
+
 template <typename R>
 int run_chunked(qsim_state *st, const qsim_op *ops, int nops, const void *mats, uint64_t mat_count)
 {
     (void)mat_count;
     Plan plan;
-    if (plan_for(st, ops, nops, mats, 1, plan)) return -1;
     std::vector<hipFunction_t> fns;
-    if (jit_for(st, plan, sizeof(R) == 8, 1, fns)) return -1;
+    if (plan_and_compile<R>(st, ops, nops, mats, plan, fns)) return -1;
     DescImage img;
     img.build<R>(plan);
     void *dev = nullptr;
@@ -358,6 +426,9 @@ int run_chunked(qsim_state *st, const qsim_op *ops, int nops, const void *mats, 
 template <typename R>
 int run_graphed(qsim_state *st, const qsim_op *ops, int nops, const void *mats, uint64_t mat_count)
 {
+    // a layout change that cannot fold is applied first, so the key sees
+    // the buffer the passes will use
+    if (st->perm_pending && !can_fold(st) && flush_perm(st)) return -1;
     // key: buffer address, geometry, ops, matrices
     std::vector<char> key;
     auto put = [&key](const void *p, size_t b) {
@@ -369,25 +440,34 @@ int run_graphed(qsim_state *st, const qsim_op *ops, int nops, const void *mats, 
     put(&st->cmin, sizeof(int));
     put(&st->jit, sizeof(bool));
     put(&st->fuse, sizeof(bool));
+    const bool fold = can_fold(st);
+    put(&fold, sizeof(bool));
+    if (fold) put(st->perm_map, sizeof(int32_t) * st->n);
     put(ops, sizeof(qsim_op) * nops);
     put(mats, sizeof(cx<R>) * mat_count);
     for (auto it = st->graphs.begin(); it != st->graphs.end(); ++it) {
         if (it->key == key) {
             st->graphs.splice(st->graphs.begin(), st->graphs, it);
             HIP_TRY(hipGraphLaunch(st->graphs.front().exec, st->stream));
+            if (st->graphs.front().swaps) {  // replayed the folded layout change
+                std::swap(st->psi, st->tmp);
+                st->perm_pending = false;
+                st->counters[6]++;
+            }
             st->counters[1]++;
             st->counters[3] += 0;
             return 0;
         }
     }
+    if (fold && ensure_tmp(st)) return -1;
     Plan plan;
-    if (plan_for(st, ops, nops, mats, 1, plan)) return -1;
     std::vector<hipFunction_t> fns;
-    if (jit_for(st, plan, sizeof(R) == 8, 1, fns)) return -1;
+    if (plan_and_compile<R>(st, ops, nops, mats, plan, fns)) return -1;
     DescImage img;
     img.build<R>(plan);
     GraphEntry e;
     e.key.swap(key);
+    e.swaps = !plan.in_map.empty();
     HIP_TRY(hipMalloc(&e.desc, img.bytes.size()));
     HIP_TRY(hipMemcpyAsync(e.desc, img.bytes.data(), img.bytes.size(), hipMemcpyHostToDevice, st->stream));
     HIP_TRY(hipStreamSynchronize(st->stream));
@@ -432,7 +512,10 @@ template <typename R>
 int apply_ops_t(qsim_state *st, int mode, const qsim_op *ops, int nops, const void *mats, uint64_t mat_count)
 {
     if (nops == 0) return 0;
-    if (mode == QSIM_SEQUENTIAL || !chunk_ok(st)) return run_sequential<R>(st, ops, nops, mats);
+    if (mode == QSIM_SEQUENTIAL || !chunk_ok(st)) {
+        if (flush_perm(st)) return -1;
+        return run_sequential<R>(st, ops, nops, mats);
+    }
     if (mode == QSIM_COOPERATIVE) return run_chunked<R>(st, ops, nops, mats, mat_count);
     if (mode == QSIM_GRAPHED) return run_graphed<R>(st, ops, nops, mats, mat_count);
     return fail("apply_tile: unknown tile mode " + std::to_string(mode));
@@ -630,7 +713,9 @@ int qsim_create(int num_qubits, int dtype, qsim_state **out)
     st->n = num_qubits;
     st->dtype = dtype;
     st->dim = 1ull << num_qubits;
-    st->K = dtype == QSIM_C64 ? 12 : 11;
+    // chunk bits: complex64 2^13 amplitudes (64 KB LDS, 512-thread blocks),
+    // complex128 2^11 (32 KB)
+    st->K = dtype == QSIM_C64 ? 13 : 11;
     st->cmin = dtype == QSIM_C64 ? 4 : 3;
     if (const char *e = std::getenv("QSIM_JIT")) st->jit = std::atoi(e) != 0;
     HIP_TRY(hipGetDevice(&st->device));
@@ -681,6 +766,7 @@ int qsim_dtype(const qsim_state *st) { return st ? st->dtype : -1; }
 int qsim_device_ptr(qsim_state *st, void **out)
 {
     if (!st || !out) return fail("null argument");
+    if (flush_perm(st)) return -1;
     *out = st->psi;
     return 0;
 }
@@ -688,6 +774,7 @@ int qsim_device_ptr(qsim_state *st, void **out)
 int qsim_reset(qsim_state *st)
 {
     if (!st) return fail("null state");
+    st->perm_pending = false;  // the state is overwritten
     if (st->dtype == QSIM_C64)
         HIP_TRY(launch_init_basis<float>(static_cast<cx<float> *>(st->psi), st->dim, 0, st->stream));
     else
@@ -700,6 +787,7 @@ int qsim_from_host(qsim_state *st, const void *src, uint64_t count)
 {
     if (!st) return fail("null state");
     if (count != st->dim) return fail("Input array must have size 2^num_qubits");
+    st->perm_pending = false;  // the state is overwritten
     HIP_TRY(hipMemcpyAsync(st->psi, src, count * amp_bytes(st->dtype), hipMemcpyHostToDevice, st->stream));
     HIP_TRY(hipStreamSynchronize(st->stream));
     return 0;
@@ -709,6 +797,7 @@ int qsim_to_host(qsim_state *st, void *dst, uint64_t count)
 {
     if (!st) return fail("null state");
     if (count != st->dim) return fail("Output array must have size 2^num_qubits");
+    if (flush_perm(st)) return -1;
     HIP_TRY(hipMemcpyAsync(dst, st->psi, count * amp_bytes(st->dtype), hipMemcpyDeviceToHost, st->stream));
     HIP_TRY(hipStreamSynchronize(st->stream));
     return 0;
@@ -718,6 +807,7 @@ int qsim_get_amplitude(qsim_state *st, uint64_t index, double *re, double *im)
 {
     if (!st || !re || !im) return fail("null argument");
     if (index >= st->dim) return fail("amplitude index out of range");
+    if (flush_perm(st)) return -1;
     if (st->dtype == QSIM_C64) {
         float h[2];
         HIP_TRY(hipMemcpyAsync(h, static_cast<char *>(st->psi) + 8 * index, 8, hipMemcpyDeviceToHost, st->stream));
@@ -737,6 +827,7 @@ int qsim_get_amplitude(qsim_state *st, uint64_t index, double *re, double *im)
 int qsim_synchronize(qsim_state *st)
 {
     if (!st) return fail("null state");
+    if (flush_perm(st)) return -1;  // all requested work, a pending layout change included
     HIP_TRY(hipStreamSynchronize(st->stream));
     return 0;
 }
@@ -754,22 +845,17 @@ int qsim_permute(qsim_state *st, const int32_t *layout_old, const int32_t *layou
     if (num_qubits != st->n) return fail("layout_old/layout_new must have length num_qubits");
     int32_t map[64];
     if (permute_map_impl(st->n, layout_old, layout_new, map)) return -1;
+    // Recorded, not run: the next op list folds it into its first chunk pass
+    // (or it runs as one permutation pass when anything else reads the state).
+    if (st->perm_pending) {
+        int32_t comp[64];
+        for (int p = 0; p < st->n; ++p) comp[p] = map[st->perm_map[p]];
+        std::memcpy(map, comp, sizeof(int32_t) * st->n);
+    }
     bool ident = true;
     for (int p = 0; p < st->n; ++p) ident &= map[p] == p;
-    if (ident) return 0;
-    if (ensure_tmp(st)) return -1;
-    PermArgs P;
-    build_perm_args(st->n, map, st->dtype == QSIM_C64 ? 12 : 11, P);
-    ProfScope ps(st, 1);
-    if (st->dtype == QSIM_C64)
-        HIP_TRY(launch_permute<float>(static_cast<cx<float> *>(st->psi), static_cast<cx<float> *>(st->tmp), P,
-                                      st->stream));
-    else
-        HIP_TRY(launch_permute<double>(static_cast<cx<double> *>(st->psi), static_cast<cx<double> *>(st->tmp), P,
-                                       st->stream));
-    std::swap(st->psi, st->tmp);
-    st->counters[0]++;
-    st->counters[4]++;
+    st->perm_pending = !ident;
+    std::memcpy(st->perm_map, map, sizeof(int32_t) * st->n);
     return 0;
 }
 
@@ -790,6 +876,7 @@ int qsim_apply_1q(qsim_state *st, int target_qubit, const void *U)
 {
     if (!st) return fail("null state");
     if (target_qubit < 0 || target_qubit >= st->n) return fail("Invalid target qubit");
+    if (flush_perm(st)) return -1;
     if (st->dtype == QSIM_C64) {
         Mat1<float> M;
         std::memcpy(M.u, U, sizeof(M));
@@ -808,6 +895,7 @@ int qsim_apply_2q(qsim_state *st, int q0, int q1, const void *U)
     if (!st) return fail("null state");
     if (q0 < 0 || q0 >= st->n || q1 < 0 || q1 >= st->n) return fail("Invalid qubit indices");
     if (q0 == q1) return fail("q0 and q1 must be different");
+    if (flush_perm(st)) return -1;
     if (st->dtype == QSIM_C64) {
         Mat2<float> M;
         std::memcpy(M.u, U, sizeof(M));
@@ -825,6 +913,7 @@ int qsim_apply_diagonal_1q(qsim_state *st, int q, double p0r, double p0i, double
 {
     if (!st) return fail("null state");
     if (q < 0 || q >= st->n) return fail("Invalid target qubit");
+    if (flush_perm(st)) return -1;
     if (st->dtype == QSIM_C64)
         HIP_TRY(launch_diag1q<float>(static_cast<cx<float> *>(st->psi), st->n, q, cx<float>{(float)p0r, (float)p0i},
                                      cx<float>{(float)p1r, (float)p1i}, st->stream));
@@ -881,6 +970,7 @@ int qsim_apply_2q_raw(void *psi, int dtype, int n, int q0, int q1, const void *U
 int qsim_norm2(qsim_state *st, double *out)
 {
     if (!st || !out) return fail("null argument");
+    if (flush_perm(st)) return -1;
     if (st->dtype == QSIM_C64) return dot_t<float>(st, st->psi, st->psi, out, nullptr);
     return dot_t<double>(st, st->psi, st->psi, out, nullptr);
 }
@@ -889,6 +979,7 @@ int qsim_overlap(qsim_state *a, qsim_state *b, double *re, double *im)
 {
     if (!a || !b || !re || !im) return fail("null argument");
     if (a->n != b->n || a->dtype != b->dtype) return fail("overlap: states differ in size or precision");
+    if (flush_perm(a) || flush_perm(b)) return -1;
     HIP_TRY(hipStreamSynchronize(b->stream));
     if (a->dtype == QSIM_C64) return dot_t<float>(a, a->psi, b->psi, re, im);
     return dot_t<double>(a, a->psi, b->psi, re, im);
@@ -897,6 +988,7 @@ int qsim_overlap(qsim_state *a, qsim_state *b, double *re, double *im)
 int qsim_scale(qsim_state *st, double factor)
 {
     if (!st) return fail("null state");
+    if (flush_perm(st)) return -1;
     if (st->dtype == QSIM_C64)
         HIP_TRY(launch_scale<float>(static_cast<cx<float> *>(st->psi), st->dim, factor, st->stream));
     else
@@ -910,6 +1002,7 @@ int qsim_expect_pauli(qsim_state *st, const qsim_pauli *terms, int nterms, doubl
     if (!st) return fail("null state");
     if (nterms < 0 || (nterms > 0 && !terms)) return fail("expect_pauli: bad term array");
     if (st->n < 6) return fail("expect_pauli: needs at least 6 qubits");
+    if (flush_perm(st)) return -1;
     if (st->dtype == QSIM_C64) return pauli_t<float>(st, terms, nterms, nullptr, re, im);
     return pauli_t<double>(st, terms, nterms, nullptr, re, im);
 }
@@ -920,6 +1013,7 @@ int qsim_adjoint(qsim_state *st, const qsim_pauli *terms, int nterms, const qsim
     if (!st) return fail("null state");
     if (st->n < 6) return fail("adjoint: needs at least 6 qubits");
     if (nprobes < 0 || (nprobes > 0 && !probes_out)) return fail("adjoint: bad probe output");
+    if (flush_perm(st)) return -1;
     if (check_ops(st->n, rev_ops, nops, mat_count, true)) return -1;
     for (int i = 0; i < nops; ++i)
         if (rev_ops[i].kind == QSIM_OP_PROBE && (rev_ops[i].q1 < 0 || rev_ops[i].q1 >= nprobes))
@@ -956,7 +1050,7 @@ int qsim_set_fusion(qsim_state *st, int mode)
 }
 
 int qsim_jit_compile_check(int num_qubits, int dtype, const qsim_op *ops, int nops, const void *mats,
-                           uint64_t mat_count, int K, int cmin, int fuse)
+                           uint64_t mat_count, int K, int cmin, int fuse, const int32_t *in_map)
 {
     if (dtype != QSIM_C64 && dtype != QSIM_C128) return fail("unknown dtype");
     if (check_ops(num_qubits, ops, nops, mat_count, true)) return -1;
@@ -967,7 +1061,8 @@ int qsim_jit_compile_check(int num_qubits, int dtype, const qsim_op *ops, int no
     Plan plan;
     std::string err;
     if (!plan_passes(num_qubits, pops.data(), nops, mats, (int)amp_bytes(dtype), K, cmin,
-                     nv == 2 ? kMaxProbes : 1, plan, err, fuse != 0 && nv == 1))
+                     nv == 2 ? kMaxProbes : 1, plan, err, fuse != 0 && nv == 1,
+                     nv == 1 && dtype == QSIM_C64 ? in_map : nullptr))
         return fail(err);
     if (!jit_compile_only(plan, dtype == QSIM_C128, nv, "gfx950", err))
         return fail("run-time kernel compile: " + err);
@@ -1007,7 +1102,7 @@ int qsim_profile_read(qsim_state *st, double *ms_out, int64_t *count_out, int nc
 int qsim_get_counters(const qsim_state *st, int64_t *out, int n)
 {
     if (!st || !out) return fail("null argument");
-    const int m = std::min(n, 6);
+    const int m = std::min(n, 7);
     for (int i = 0; i < m; ++i) out[i] = st->counters[i];
     return m;
 }

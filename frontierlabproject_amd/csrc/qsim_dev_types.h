@@ -18,6 +18,7 @@ constexpr int kTPB = 256;              // threads per block for every kernel
 constexpr int kWaves = kTPB / 64;
 
 constexpr int kMaxK = 14;      // chunk bits
+constexpr int kMaxKFold = 13;  // chunk bits of a pass that folds a layout change (complex64 LDS)
 constexpr int kG = 4;          // register-step bits
 constexpr int kMaxComp = 50;   // bits outside a chunk
 #ifndef QSIM_ROUNDS

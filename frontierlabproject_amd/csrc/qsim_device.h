@@ -424,6 +424,38 @@ __device__ __forceinline__ uint64_t chunk_base(uint64_t c, const PassDev *__rest
     return base;
 }
 
+// Copy-in of a complex64 chunk whose source is in another layout (a layout
+// change folded into the pass).  Read index r enumerates the chunk in source
+// order: rows of 2^cf contiguous source amplitudes at base + rowoff[r >> cf];
+// sigma(r) is the amplitude's chunk index in the pass layout, and bit jb0 of
+// it is source bit 0, so the two amplitudes of one 16-byte read land at
+// sigma(r) and sigma(r) | 2^jb0.  Needs cf >= 1.
+template <int TPB, typename Sigma>
+__device__ __forceinline__ void chunk_copy_in_mapped(const cx<float> *__restrict__ g, cx<float> *__restrict__ lds,
+                                                     uint64_t base, const uint64_t *__restrict__ rowoff, int E,
+                                                     int cf, Sigma sigma, int jb0)
+{
+    const int lowmask = (1 << cf) - 1, nv = E >> 1;
+    for (int v0 = 0; v0 < nv; v0 += TPB * kBatch) {
+        v4f buf[kBatch];
+#pragma unroll
+        for (int j = 0; j < kBatch; ++j) {
+            const int v = v0 + j * TPB + threadIdx.x;
+            const int r = v << 1;
+            if (v < nv) buf[j] = gload<true>(reinterpret_cast<const v4f *>(g + base + rowoff[r >> cf] + (r & lowmask)));
+        }
+#pragma unroll
+        for (int j = 0; j < kBatch; ++j) {
+            const int v = v0 + j * TPB + threadIdx.x;
+            if (v < nv) {
+                const int e = sigma(v << 1);
+                lds[swz(e)] = cx<float>{buf[j].x, buf[j].y};
+                lds[swz(e | (1 << jb0))] = cx<float>{buf[j].z, buf[j].w};
+            }
+        }
+    }
+}
+
 // The register steps of one pass over the chunk staged in LDS (c0, and c1 for
 // the second vector).  Ends with a barrier after every step.
 template <typename R, int NV>

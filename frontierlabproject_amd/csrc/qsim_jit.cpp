@@ -228,8 +228,83 @@ bool compile_all(const std::vector<std::string> &bodies, const std::string &arch
 
 }  // namespace
 
+namespace {
+
+// Field moves "| (((x >> from) & mask) << to)" for a bit map src k -> dst[k],
+// grouping runs of consecutive sources with consecutive destinations.
+std::string bit_moves(const char *x, const int *dst, int nbits, const char *ty)
+{
+    std::ostringstream o;
+    for (int k = 0; k < nbits;) {
+        int len = 1;
+        while (k + len < nbits && dst[k + len] == dst[k] + len) ++len;
+        const unsigned long long mask = (1ull << len) - 1;
+        o << " | ((((" << ty << ")" << x << " >> " << k << ") & " << mask << "ull) << " << dst[k] << ")";
+        k += len;
+    }
+    return o.str();
+}
+
+// First pass of a plan with a folded layout change: reads `psi` in the
+// source layout, writes `out` in the pass layout (complex64, one vector).
+std::string mapped_pass_source(const Plan &plan, const std::string &name)
+{
+    const PassDev &pd = plan.passes[0];
+    const int n = (int)plan.in_map.size();
+    const int K = pd.K, CF = pd.cfast, E = 1 << K, NROW = 1 << (K - CF);
+    const int TPB = jit_block_threads(K, 1);
+    int inv[64];
+    for (int q = 0; q < n; ++q) inv[plan.in_map[q]] = q;
+    // chunk bits in source order: source position, chunk-local index
+    std::vector<std::pair<int, int>> src;
+    for (int j = 0; j < K; ++j) src.push_back({inv[pd.pos[j]], j});
+    std::sort(src.begin(), src.end());
+    int J[64], Pold[64];
+    for (int k = 0; k < K; ++k) Pold[k] = src[k].first, J[k] = src[k].second;
+    int CFR = 0;
+    while (CFR < K && Pold[CFR] == CFR) ++CFR;
+    const int NROWIN = 1 << (K - CFR);
+    int comp_old[64];
+    for (int m = 0; m < pd.ncomp; ++m) comp_old[m] = inv[pd.comp[m]];
+    std::ostringstream o;
+    o << "extern \"C\" __global__ __launch_bounds__(" << TPB << ") __attribute__((amdgpu_waves_per_eu(4)))\nvoid "
+      << name << "(const cx<float> *__restrict__ psi, cx<float> *__restrict__ out, "
+      << "const cx<float> *__restrict__ mats)\n{\n";
+    o << "  __shared__ __align__(16) cx<float> c0[" << E << "];\n";
+    o << "  __shared__ uint64_t rowoff[" << NROW << "];\n  __shared__ uint64_t rowin[" << NROWIN << "];\n";
+    o << "  for (int t = threadIdx.x; t < " << NROW << "; t += " << TPB << ")\n    rowoff[t] = 0ull"
+      << bit_moves("t", pd.pos + CF, K - CF, "uint64_t") << ";\n";
+    o << "  for (int t = threadIdx.x; t < " << NROWIN << "; t += " << TPB << ")\n    rowin[t] = 0ull"
+      << bit_moves("t", Pold + CFR, K - CFR, "uint64_t") << ";\n";
+    o << "  const uint64_t c = xcd_slab(blockIdx.x, " << pd.ncomp << ");\n";
+    o << "  const uint64_t base = 0ull" << bit_moves("c", pd.comp, pd.ncomp, "uint64_t") << ";\n";
+    o << "  const uint64_t base_in = 0ull" << bit_moves("c", comp_old, pd.ncomp, "uint64_t") << ";\n";
+    o << "  auto sigma = [](int r) -> int { return 0" << bit_moves("r", J, K, "unsigned") << "; };\n";
+    o << "  __syncthreads();\n";
+    o << "  chunk_copy_in_mapped<" << TPB << ">(psi, c0, base_in, rowin, " << E << ", " << CFR << ", sigma, " << J[0]
+      << ");\n";
+    o << "  __syncthreads();\n";
+    for (int s = 0; s < pd.nsteps; ++s) emit_step(o, plan.steps[pd.step_off + s], E, "float", 1, TPB);
+    o << "  chunk_copy<float, false, true, " << TPB << ">(out, c0, base, rowoff, " << E << ", " << CF << ");\n}\n\n";
+    return o.str();
+}
+
+}  // namespace
+
+bool jit_mapped_first(const Plan &plan, bool dbl, int nv)
+{
+    if (plan.in_map.empty() || dbl || nv != 1 || plan.passes.empty()) return false;
+    const PassDev &pd = plan.passes[0];
+    int inv[64];
+    for (size_t q = 0; q < plan.in_map.size(); ++q) inv[plan.in_map[q]] = (int)q;
+    bool has0 = false;  // source bit 0 is a chunk bit (16-byte reads)
+    for (int j = 0; j < pd.K; ++j) has0 |= inv[pd.pos[j]] == 0;
+    return has0 && pd.K >= 2;
+}
+
 std::string jit_pass_source(const Plan &plan, size_t p, bool dbl, int nv, const std::string &name)
 {
+    if (p == 0 && jit_mapped_first(plan, dbl, nv)) return mapped_pass_source(plan, name);
     const PassDev &pd = plan.passes[p];
     const char *T = dbl ? "double" : "float";
     const int K = pd.K, CF = pd.cfast, E = 1 << K, NROW = 1 << (K - CF);

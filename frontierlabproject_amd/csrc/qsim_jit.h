@@ -41,4 +41,9 @@ long jit_compiled_count();
 // Threads per block of the specialised kernel of a pass with K chunk bits.
 int jit_block_threads(int K, int nv);
 
+// Whether the first pass of `plan` folds its layout change (Plan::in_map):
+// its kernel is then kernel(psi_in, psi_out, mats), reading the source layout
+// and writing the pass layout to the other buffer (complex64, one vector).
+bool jit_mapped_first(const Plan &plan, bool dbl, int nv);
+
 }  // namespace qsim

@@ -316,7 +316,7 @@ void to_dense(Ctx &c, StepBuilder &sb)
 }  // namespace
 
 bool plan_passes(int n, const PlanOp *ops, int nops, const void *mats, int elem_bytes, int K, int cmin,
-                 int max_probes, Plan &out, std::string &err, bool fuse)
+                 int max_probes, Plan &out, std::string &err, bool fuse, const int32_t *in_map)
 {
     out = Plan();
     if (n < kG || n > 63) {
@@ -360,9 +360,30 @@ bool plan_passes(int n, const PlanOp *ops, int nops, const void *mats, int elem_
     for (int i = 0; i < nops; ++i) any_probe |= ops[i].kind == OP_PROBE;
     Cd pending = {1.0, 0.0};  // scalar factored out of finished steps, not yet applied
 
-    while (!rem.empty()) {
+    if (in_map) {
+        uint64_t seen = 0;
+        for (int p = 0; p < n; ++p) {
+            if (in_map[p] < 0 || in_map[p] >= n || ((seen >> in_map[p]) & 1)) {
+                err = "tile plan: input map is not a permutation";
+                return false;
+            }
+            seen |= 1ull << in_map[p];
+        }
+        out.in_map.assign(in_map, in_map + n);
+    }
+    bool first = true;  // a layout change with no op still gets its (stepless) pass
+    while (!rem.empty() || (first && in_map)) {
         // ---- pass: chunk bit set ----
         uint64_t P = low, blocked = 0;
+        // a folded layout change reserves the images of the low input bits;
+        // that pass gets one more chunk bit for them (when the state has it)
+        int Kp = K;
+        if (first && in_map) {
+            for (int k = 0; k < cmin; ++k) P |= 1ull << in_map[k];
+            Kp = std::min(std::min(K + 1, n), kMaxKFold);
+            if (Kp < K) Kp = K;
+        }
+        first = false;
         int nprobe = 0;
         std::vector<int> chosen, left;
         for (int idx : rem) {
@@ -370,7 +391,7 @@ bool plan_passes(int n, const PlanOp *ops, int nops, const void *mats, int elem_
             uint64_t m = 1ull << op.q0;
             if (op.kind == OP_2Q) m |= 1ull << op.q1;
             const bool probe = op.kind == OP_PROBE;
-            if ((m & blocked) || popc(P | m) > K || (probe && nprobe >= max_probes)) {
+            if ((m & blocked) || popc(P | m) > Kp || (probe && nprobe >= max_probes)) {
                 blocked |= m;
                 left.push_back(idx);
                 continue;
@@ -379,10 +400,10 @@ bool plan_passes(int n, const PlanOp *ops, int nops, const void *mats, int elem_
             nprobe += probe;
             chosen.push_back(idx);
         }
-        P = fill_low(P, n, K);
+        P = fill_low(P, n, Kp);
         PassDev pd;
         int loc[64];
-        make_pass(n, K, P, pd, loc);
+        make_pass(n, Kp, P, pd, loc);
         pd.step_off = (int32_t)out.steps.size();
         pd.probe_off = (int32_t)out.probe_ids.size();
         pd.nprobe = 0;

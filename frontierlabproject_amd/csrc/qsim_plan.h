@@ -47,6 +47,11 @@ struct Plan {
     std::vector<int32_t> probe_ids;  // per pass, slots in order (probe_off indexes here)
     std::vector<double> mats;        // re, im pairs (converted to the state precision on upload)
     std::vector<int32_t> order;      // op indices in execution order
+    // When set (a layout change folded into the first pass): in_map[p] is the
+    // position in the pass layout of input bit position p.  The first pass
+    // then reads the input in the old layout and writes its output, in the
+    // pass layout, to another buffer.
+    std::vector<int32_t> in_map;
 };
 
 // Scalar factors (a diagonal gate's first entry, H's 1/sqrt2) are taken out of
@@ -60,8 +65,12 @@ struct Plan {
 // Returns false and sets err on invalid input.
 // fuse: every register step becomes one dense 16x16 matrix, the product of
 // all its ops (plans without probes only).
+// in_map (n entries, or null): the input is in another layout (see
+// Plan::in_map); the first pass also holds the images of the lowest cmin
+// input bit positions, so it reads runs of 2^cmin contiguous amplitudes.
 bool plan_passes(int n, const PlanOp *ops, int nops, const void *mats, int elem_bytes, int K,
-                 int cmin, int max_probes, Plan &out, std::string &err, bool fuse = false);
+                 int cmin, int max_probes, Plan &out, std::string &err, bool fuse = false,
+                 const int32_t *in_map = nullptr);
 
 // Pauli chunk passes: each term's X support must lie inside its pass's chunk.
 struct PauliPass {

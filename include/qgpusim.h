@@ -81,7 +81,8 @@ int qsim_to_host(qsim_state *st, void *dst, uint64_t count);
  * reference counterpart (used for fidelity checks without a full copy). */
 int qsim_get_amplitude(qsim_state *st, uint64_t index, double *re, double *im);
 
-/* synchronize() (bindings.cpp:585-587): waits for the state's stream. */
+/* synchronize() (bindings.cpp:585-587): applies a pending layout change,
+ * then waits for the state's stream. */
 int qsim_synchronize(qsim_state *st);
 
 /* Host only, no device work: map[p_old] = new position of the qubit held at
@@ -89,8 +90,14 @@ int qsim_synchronize(qsim_state *st);
 int qsim_permute_map(int num_qubits, const int32_t *layout_old, const int32_t *layout_new,
                      int32_t *map_out);
 
-/* permute(layout_old, layout_new) (bindings.cpp:434-483): one LDS-tiled
- * gather/scatter pass into the second buffer, then the buffers swap. */
+/* permute(layout_old, layout_new) (bindings.cpp:434-483), with the same map
+ * and errors.  The layout change is recorded (composed with one still
+ * pending) and applied lazily: the next qsim_apply_ops in a chunked mode on a
+ * complex64 state with specialised kernels folds it into its first chunk
+ * pass, which reads the old layout and writes the new one into the second
+ * buffer; any other use of the state first applies it as one LDS-tiled
+ * gather/scatter pass into the second buffer, after which the buffers swap.
+ * Results are those of an immediate permutation. */
 int qsim_permute(qsim_state *st, const int32_t *layout_old, const int32_t *layout_new, int num_qubits);
 
 /* Host only: the integer checks of parse_tile_ops (bindings.cpp:196-216). */
@@ -161,11 +168,13 @@ int qsim_set_jit(qsim_state *st, int enable);
 
 /* Host only, no reference counterpart: plan `ops` as qsim_apply_ops would (as
  * the adjoint's reverse sweep would, on psi and lam, when `ops` hold
- * QSIM_OP_PROBE entries) and compile, without loading, every specialised pass
- * kernel for gfx950.  Returns the number of passes, or -1 with the compiler
- * log. */
+ * QSIM_OP_PROBE entries; with a pending layout change in_map[p] = new
+ * position of bit p, complex64 forward only, or NULL) and compile, without
+ * loading, every specialised pass kernel for gfx950.  Returns the number of
+ * passes, or -1 with the compiler log. */
 int qsim_jit_compile_check(int num_qubits, int dtype, const qsim_op *ops, int nops,
-                           const void *mats, uint64_t mat_count, int K, int cmin, int fuse);
+                           const void *mats, uint64_t mat_count, int K, int cmin, int fuse,
+                           const int32_t *in_map);
 
 /* No reference counterpart.  Fusion mode of forward chunk passes:
  * 0 (default): slot programs (each gate by its own cheap arithmetic);
@@ -186,8 +195,9 @@ int qsim_profile(qsim_state *st, int enable);
 int qsim_profile_read(qsim_state *st, double *ms_out, int64_t *count_out, int nclasses);
 
 /* Counters: [0] kernel launches, [1] graph launches, [2] graph captures,
- * [3] chunk passes, [4] permutations, [5] chunk passes run by specialised
- * kernels.  Returns the number written. */
+ * [3] chunk passes, [4] permutation passes run, [5] chunk passes run by
+ * specialised kernels, [6] layout changes folded into a chunk pass.
+ * Returns the number written. */
 int qsim_get_counters(const qsim_state *st, int64_t *out, int n);
 
 /* Host only: plan `ops` into chunk passes without a device.  Writes, per pass,

@@ -484,3 +484,89 @@ def test_fused_steps_match_oracle(hip, dtype, K, circ):
         _check(ref, got, dtype)
         outs.append(got)
     assert dense.fidelity(outs[0], outs[1]) > 1 - TOL[dtype]
+
+
+def _layouts(n, rng, k):
+    out = [list(range(n))]
+    for _ in range(k):
+        out.append([int(x) for x in rng.permutation(n)])
+    return out
+
+
+@pytest.mark.parametrize("dtype", [np.complex64, np.complex128])
+@pytest.mark.parametrize("jit", [True, False])
+@pytest.mark.parametrize("mode", ["apply_tile_cooperative", "apply_tile_graphed", "apply_tile_dev"])
+def test_lazy_permutations_match_oracle(hip, dtype, jit, mode):
+    """permute() is applied lazily: folded into the next op list's first chunk
+    pass (complex64, specialised kernels) or run as a permutation pass.  Two
+    layout changes in a row compose; the result equals immediate permutation
+    followed by the gates, and a second identical run (graph replay) repeats it."""
+    n = 14
+    rng = np.random.default_rng(21)
+    L = _layouts(n, rng, 4)
+    ops1 = _tuples(C.random_circuit(n, 4, 31))
+    ops2 = _tuples(C.random_circuit(n, 3, 32))
+    psi0 = C.random_state(n, 9)
+    ref = P.permute(psi0, L[0], L[1])
+    for o in ops1:
+        ref = dense.apply(ref, o[-1], list(o[1:-1]), n)
+    ref = P.permute(P.permute(ref, L[1], L[2]), L[2], L[3])
+    for o in ops2:
+        ref = dense.apply(ref, o[-1], list(o[1:-1]), n)
+    ref = P.permute(ref, L[3], L[4])
+    sv = _sv(hip, n, dtype)
+    sv.set_jit(jit)
+    # three buffer swaps per run: the third run starts on the first run's
+    # buffer, so in graphed mode it replays the captured folded passes
+    for rep in range(3):
+        sv.from_numpy(psi0)
+        sv.permute(L[0], L[1])
+        getattr(sv, mode)(ops1)
+        sv.permute(L[1], L[2])
+        sv.permute(L[2], L[3])
+        getattr(sv, mode)(ops2)
+        sv.permute(L[3], L[4])
+        _check(ref, sv.to_numpy(), dtype)
+    c = sv.counters()
+    folds = dtype == np.complex64 and jit and mode != "apply_tile_dev"
+    assert (c["folded_permutations"] == 6) if folds else c["folded_permutations"] == 0
+    if folds and mode == "apply_tile_graphed":
+        assert c["graph_launches"] > c["graph_captures"]
+
+
+@pytest.mark.parametrize("dtype", [np.complex64, np.complex128])
+def test_pending_permutation_flushes(hip, dtype):
+    """Every reader of the state sees the permuted state."""
+    n = 12
+    rng = np.random.default_rng(5)
+    L = _layouts(n, rng, 2)
+    psi0 = C.random_state(n, 10)
+    ref = P.permute(psi0, L[0], L[1])
+    sv = _sv(hip, n, dtype)
+    other = _sv(hip, n, dtype)
+    other.from_numpy(ref)
+    for probe in ("amplitude", "overlap", "apply_1q", "norm", "expect"):
+        sv.from_numpy(psi0)
+        sv.permute(L[0], L[1])
+        if probe == "amplitude":
+            k = int(np.argmax(np.abs(ref)))
+            assert abs(sv.amplitude(k) - ref[k]) < 1e-6
+        elif probe == "overlap":
+            assert abs(sv.overlap(other) - 1) < 1e-5
+        elif probe == "apply_1q":
+            sv.apply_1q(3, G.x())
+            _check(dense.apply(ref, G.x(), [3], n), sv.to_numpy(), dtype)
+        elif probe == "norm":
+            assert abs(sv.norm_squared() - 1) < 1e-5
+            _check(ref, sv.to_numpy(), dtype)
+        else:
+            z = np.where(np.arange(1 << n) & 1, -1.0, 1.0)
+            e = sv.expectation([(0, 1, 1.0)]).real
+            assert abs(e - float(np.sum(z * np.abs(ref) ** 2))) < 1e-5
+    # reset discards a pending layout change
+    sv.from_numpy(psi0)
+    sv.permute(L[0], L[1])
+    sv.reset()
+    e0 = np.zeros(1 << n)
+    e0[0] = 1
+    _check(e0, sv.to_numpy(), dtype)
