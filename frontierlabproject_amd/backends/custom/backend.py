@@ -146,13 +146,18 @@ def build_tile_ops(tile: Tile, qubit_map, np_dtype):
 
 def run_custom_backend(qc, tile_plan: List[Tile], task: str, shots: int = None,
                        use_float32: bool = False, use_cuda_graphs: bool = True,
-                       tile_mode: str = "cooperative", enable_fusion: bool = True):
+                       tile_mode: str = "cooperative", enable_fusion: bool = True, sample_seed: int = 0):
     """Simulate ``qc`` along ``tile_plan`` on the GPU (backend.py:336-522).
 
     Returns ``(statevector, metrics)``; ``metrics`` has the reference's keys:
     num_tiles, num_gates, num_fused_ops, fusion_ratio, num_reorders,
     num_kernel_launches, num_graph_replays, avoided_launches, t_perm_s,
     t_apply_s, t_sync_s, t_total_s, t_launch_overhead_saved_us, tile_mode.
+    ``permute`` is lazy (folded into the next tile's first chunk pass), so
+    t_perm_s holds only its host side and the permutation work is in
+    t_apply_s.  With ``task="sampling"`` and ``shots``, ``metrics["counts"]``
+    holds outcomes sampled on the GPU from the final state (seed
+    ``sample_seed``) over the circuit's measurements, or all qubits.
     """
     if tile_mode not in ("cooperative", "graphed", "sequential"):
         raise ValueError(f"unknown tile_mode {tile_mode!r}")
@@ -196,6 +201,12 @@ def run_custom_backend(qc, tile_plan: List[Tile], task: str, shots: int = None,
     t_sync0 = time.perf_counter()
     sv.synchronize()
     t_sync = time.perf_counter() - t_sync0
+    counts = None
+    if task == "sampling" and shots:
+        from ...sampling import counts_from_indices, measurements_of
+        meas = measurements_of(qc)
+        nbits = max(getattr(qc, "num_clbits", 0) or 0, 1 + max(c for _, c in meas))
+        counts = counts_from_indices(sv.sample(int(shots), seed=sample_seed), layout, meas, nbits)
     result = sv.to_numpy()
     counters = sv.counters()
     del sv
@@ -228,4 +239,6 @@ def run_custom_backend(qc, tile_plan: List[Tile], task: str, shots: int = None,
         "device_graph_launches": counters["graph_launches"],
         "device_chunk_passes": counters["chunk_passes"],
     }
+    if counts is not None:
+        metrics["counts"] = counts
     return result, metrics

@@ -85,6 +85,8 @@ _SIGNATURES = {
     "qsim_batch_info": ([_P, _P, _I], _I),
     "qsim_jit_compile_check": ([_I, _I, _P, _I, _P, _U64, _I, _I, _I, _P], _I),
     "qsim_set_fusion": ([_P, _I], _I),
+    "qsim_device_memory": ([_P, _P], _I),
+    "qsim_sample": ([_P, _U64, _U64, _P], _I),
 }
 
 EXPORTED = tuple(_SIGNATURES)
@@ -330,6 +332,13 @@ class _StatevectorBase:
     def set_tile_params(self, K, cmin):
         _check(lib.qsim_set_tile_params(self._h, int(K), int(cmin)))
 
+    def sample(self, shots, seed=0):
+        """`shots` outcomes drawn from |psi|^2 on the GPU (FP64): amplitude
+        indices in the current bit order, grouped by index block."""
+        out = np.zeros(max(1, int(shots)), dtype=np.uint64)
+        _check(lib.qsim_sample(self._h, int(shots), int(seed) & (2 ** 64 - 1), out.ctypes.data))
+        return out[:int(shots)]
+
     def set_fusion(self, mode):
         """0: slot programs; 1: every register step one fused 4-qubit unitary on
         the matrix cores (complex64, specialised kernels)."""
@@ -435,6 +444,13 @@ def validate_ops(ops, num_qubits):
     """Host-only integer checks of the C layer on packed ops."""
     arr, arena = pack_ops(ops, num_qubits, np.complex64)
     _check(lib.qsim_validate_ops(num_qubits, ctypes.addressof(arr), len(ops), arena.size))
+
+
+def device_memory():
+    """(free, total) bytes of the current device."""
+    f, t = ctypes.c_uint64(), ctypes.c_uint64()
+    _check(lib.qsim_device_memory(ctypes.byref(f), ctypes.byref(t)))
+    return f.value, t.value
 
 
 def jit_compile_check(num_qubits, arr, nops, arena, K, cmin, fuse=False, in_map=None):

@@ -96,6 +96,18 @@ class SaveStatevector(Instruction):
         super().__init__("save_statevector", num_qubits)
 
 
+class Measure(Instruction):
+    """Computational-basis measurement of one qubit into one classical bit."""
+
+    def __init__(self):
+        super().__init__("measure", 1)
+
+
+class Barrier(Instruction):
+    def __init__(self, num_qubits):
+        super().__init__("barrier", num_qubits)
+
+
 def _rx(t):
     c, s = math.cos(t / 2), math.sin(t / 2)
     return np.array([[c, -1j * s], [-1j * s, c]], dtype=np.complex128)
@@ -192,6 +204,7 @@ class QuantumCircuit:
         self.data: List[CircuitInstruction] = []
         self.global_phase = float(global_phase)
         self._index = {q: i for i, q in enumerate(self.qubits)}
+        self._nclbits = 0
 
     @property
     def num_qubits(self):
@@ -248,6 +261,7 @@ class QuantumCircuit:
         c.qubits = self.qubits
         c._index = self._index
         c.data = list(self.data)
+        c._nclbits = self._nclbits
         return c
 
     def inverse(self):
@@ -264,6 +278,37 @@ class QuantumCircuit:
         for ci in self.data:
             out[ci.operation.name] = out.get(ci.operation.name, 0) + 1
         return out
+
+    def depth(self):
+        """Layers of the circuit: each instruction sits one layer above the
+        latest earlier instruction sharing a qubit or classical bit with it;
+        barriers take no layer (Qiskit's default)."""
+        level = {}
+        d = 0
+        for ci in self.data:
+            wires = [("q", self._index[q]) for q in ci.qubits] + [("c", c) for c in ci.clbits]
+            top = max((level.get(w, 0) for w in wires), default=0)
+            if ci.operation.name == "barrier":
+                for w in wires:
+                    level[w] = top
+                continue
+            for w in wires:
+                level[w] = top + 1
+            d = max(d, top + 1)
+        return d
+
+    @property
+    def num_clbits(self):
+        return self._nclbits
+
+    def measure_all(self):
+        """Barrier, then a measurement of every qubit into a new classical
+        bit (bit i <- qubit i), as Qiskit's measure_all."""
+        base = self._nclbits
+        self._nclbits += self.num_qubits
+        self.append(Barrier(self.num_qubits), self.qubits)
+        for i, q in enumerate(self.qubits):
+            self.data.append(CircuitInstruction(Measure(), (q,), (base + i,)))
 
     def __len__(self):
         return len(self.data)
@@ -303,6 +348,7 @@ class DAGCircuit:
         d.qubits = list(self.qubits)
         d.clbits = list(self.clbits)
         d.global_phase = self.global_phase
+        d.num_clbits = getattr(self, "num_clbits", 0)
         return d
 
     def apply_operation_back(self, op, qargs=(), cargs=()):
@@ -320,6 +366,7 @@ def circuit_to_dag(qc: QuantumCircuit) -> DAGCircuit:
     d.global_phase = qc.global_phase
     for ci in qc.data:
         d.apply_operation_back(ci.operation, ci.qubits, ci.clbits)
+    d.num_clbits = qc.num_clbits
     return d
 
 
@@ -329,4 +376,6 @@ def dag_to_circuit(dag: DAGCircuit) -> QuantumCircuit:
     qc._index = {q: i for i, q in enumerate(qc.qubits)}
     for node in dag.topological_op_nodes():
         qc.data.append(CircuitInstruction(node.op, node.qargs, node.cargs))
+    used = [int(c) for node in dag.topological_op_nodes() for c in node.cargs]
+    qc._nclbits = max(getattr(dag, "num_clbits", 0), 1 + max(used) if used else 0)
     return qc

@@ -185,6 +185,20 @@ int qsim_jit_compile_check(int num_qubits, int dtype, const qsim_op *ops, int no
  * the state is complex64, else by the vector units. */
 int qsim_set_fusion(qsim_state *st, int mode);
 
+/* No reference counterpart (its custom backend has no sampling task):
+ * `shots` computational-basis outcomes drawn from |psi|^2 / <psi|psi>, as
+ * amplitude indices in the state's current bit order, grouped by index
+ * block (not in draw order).  Two levels, all in FP64: per-block
+ * probability sums over blocks of 2^13 amplitudes on the GPU; each shot's
+ * block and residual from a splitmix64 stream keyed by `seed` on the host;
+ * the exact index by a binary search over the block's prefix sums in LDS.
+ * Applies a pending layout change first.  At most 2^31 shots per call. */
+int qsim_sample(qsim_state *st, uint64_t shots, uint64_t seed, uint64_t *indices_out);
+
+/* No reference counterpart (the reference reads NVML): free and total memory
+ * of the current device (hipMemGetInfo). */
+int qsim_device_memory(uint64_t *free_bytes, uint64_t *total_bytes);
+
 /* Per-kernel-class timing with HIP events recorded on the state's stream around
  * each launch (not inside captured graphs).  Classes: 0 chunk pass,
  * 1 permutation, 2 gate stream, 3 Pauli chunk pass. */
