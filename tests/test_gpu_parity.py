@@ -653,11 +653,12 @@ def test_log_run_rows(tmp_path):
     combo = {"task": "statevector", "circuit": "random", "nqubits": n, "depth": d, "transpiler": "custom",
              "nL": 7, "backend": "custom"}
     Rz.log_run(path, W.random_circuit(n, d, 900), combo, reference=psi)
-    shots = 300000
-    p = np.abs(psi) ** 2
-    ref_counts = {format(i, "0%db" % n): float(p[i] * shots) for i in range(1 << n)}
-    Rz.log_run(path, W.random_circuit(n, d, 900), dict(combo, task="sampling", shots=shots), reference=ref_counts,
-               sample_seed=3)
+    # sampling: 8 qubits, 10^6 shots (expected TVD ~0.006 against the 0.02 rule)
+    m, shots = 8, 1000000
+    q = np.abs(dense.simulate(C.to_dense_gates(C.random_circuit(m, d, 901)), m)) ** 2
+    ref_counts = {format(i, "0%db" % m): float(q[i] * shots) for i in range(1 << m)}
+    Rz.log_run(path, W.random_circuit(m, d, 901), dict(combo, task="sampling", nqubits=m, nL=5, shots=shots),
+               reference=ref_counts, sample_seed=3)
     with open(path) as f:
         rows = list(_csv.DictReader(f))
     assert len(rows) == 2 and list(rows[0]) == Rz.FIELDNAMES
