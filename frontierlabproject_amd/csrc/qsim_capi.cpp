@@ -104,6 +104,8 @@ struct GraphEntry {
     hipGraphExec_t exec = nullptr;
     void *desc = nullptr;
     bool swaps = false;  // its first pass writes the other buffer (folded layout change)
+    bool after_pending = false;  // stored layout after the replay (perm_map)
+    int32_t after_map[64];
 };
 
 size_t amp_bytes(int dtype) { return dtype == QSIM_C64 ? 8 : 16; }
@@ -131,9 +133,12 @@ struct qsim_state {
     // permutations run, chunk passes run by specialised kernels, permutations
     // folded into a chunk pass
     int64_t counters[7] = {0, 0, 0, 0, 0, 0, 0};
-    // layout change requested by qsim_permute and not yet applied:
-    // perm_map[p] = new position of the current bit position p
-    bool perm_pending = false;
+    // The stored bit order may differ from the logical one (the layout the
+    // caller's permutations define): perm_map[p] = logical bit position held
+    // by stored bit p.  qsim_permute only composes it; op lists run on the
+    // stored order (their qubits mapped through it), moving op qubits into
+    // the low bits inside their first chunk pass; readers apply it first.
+    bool perm_pending = false;  // perm_map is not the identity
     int32_t perm_map[64];
     bool jit = true;               // specialise forward chunk passes at run time
     bool fuse = false;             // every register step one 16x16 unitary (matrix cores)
@@ -387,27 +392,111 @@ int jit_for(qsim_state *st, const Plan &plan, bool dbl, int nv, std::vector<hipF
     return 0;
 }
 
-// A pending layout change folds into the first chunk pass of the next op
-// list when that pass runs a specialised complex64 kernel.
+// A stored order that differs from the logical one is kept across op lists
+// when their chunk passes run specialised complex64 kernels (a layout change
+// then folds into a first pass); otherwise it is applied first.
 bool can_fold(const qsim_state *st) { return st->perm_pending && st->jit && st->dtype == QSIM_C64; }
 
-// Plans `ops`, folding a pending layout change into the first pass when it
-// can (else applies it first), and fetches the specialised kernels.
+// Stored bits below kLowRegion span one 2 MiB window of complex64 amplitudes:
+// chunk passes over them gather and scatter within few address-translation
+// pages, while chunks over higher bits touch up to one page per row.
+constexpr int kLowRegion = 18;
+
+// Stored position tgt[q] for each logical bit q while running `ops` on a
+// stored order that differs from the logical one.  The op qubits should sit
+// below r = max(#op qubits, kLowRegion).  Each op qubit stored at or above r
+// takes the slot of a non-op qubit below r (the row bits 0..cmin-1 first, so
+// every pass's reserved low bits hold op qubits); that qubit moves to the
+// vacated position, except that one leaving a row bit takes the lowest
+// position at or above r (displacing a non-op qubit there to a vacated
+// position): the first chunk pass then scatters to near rows only, while
+// qubits outside its chunk move with the chunk base at no cost.  Every other
+// qubit stays.
+void choose_layout(const qsim_state *st, const qsim_op *ops, int nops, int32_t *tgt)
+{
+    const int n = st->n;
+    const int32_t *at = st->perm_map;  // logical bit held by stored bit p
+    for (int p = 0; p < n; ++p) tgt[at[p]] = p;
+    uint64_t T = 0;
+    for (int i = 0; i < nops; ++i) {
+        T |= 1ull << ops[i].q0;
+        if (ops[i].kind == QSIM_OP_2Q) T |= 1ull << ops[i].q1;
+    }
+    const int r = std::min(n, std::max(__builtin_popcountll(T), kLowRegion));
+    std::vector<int> slots, in;  // ascending stored positions
+    for (int p = 0; p < r; ++p)
+        if (!((T >> at[p]) & 1)) slots.push_back(p);
+    for (int p = r; p < n; ++p)
+        if ((T >> at[p]) & 1) in.push_back(p);
+    // in.size() <= slots.size(): the op qubits number at most r
+    bool vacated[64] = {}, taken[64] = {};
+    for (int p : in) vacated[p] = true;
+    std::vector<int> movers;  // qubits to place on the vacated positions left
+    int scan = r;
+    for (size_t i = 0; i < in.size(); ++i) {
+        const int slot = slots[i], leaving = at[slot];
+        tgt[at[in[i]]] = slot;
+        if (slot < st->cmin) {
+            while (taken[scan]) ++scan;  // stays below n: at most in.size() positions are taken
+            taken[scan] = true;
+            tgt[leaving] = scan;
+            if (!vacated[scan]) movers.push_back(at[scan]);
+        } else {
+            movers.push_back(leaving);
+        }
+    }
+    size_t k = 0;
+    for (int p : in)
+        if (!taken[p]) tgt[movers[k++]] = p;
+}
+
+// Stored order after an op list: `after` (perm_map form) and whether it
+// differs from the logical order.
+struct LayoutAfter {
+    bool pending = false;
+    int32_t map[64];
+};
+
 template <typename R>
 int plan_and_compile(qsim_state *st, const qsim_op *ops, int nops, const void *mats, Plan &plan,
-                     std::vector<hipFunction_t> &fns)
+                     std::vector<hipFunction_t> &fns, LayoutAfter &after)
 {
-    const bool fold = can_fold(st);
-    if (st->perm_pending && !fold && flush_perm(st)) return -1;
-    if (plan_for(st, ops, nops, mats, 1, plan, fold ? st->perm_map : nullptr)) return -1;
+    const bool keep = can_fold(st);
+    if (st->perm_pending && !keep && flush_perm(st)) return -1;
+    after.pending = false;
+    if (!keep) {
+        if (plan_for(st, ops, nops, mats, 1, plan)) return -1;
+        return jit_for(st, plan, sizeof(R) == 8, 1, fns);
+    }
+    const int n = st->n;
+    int32_t tgt[64], in_map[64];
+    choose_layout(st, ops, nops, tgt);
+    std::vector<qsim_op> mapped(ops, ops + nops);
+    for (qsim_op &o : mapped) {
+        o.q0 = tgt[o.q0];
+        if (o.kind == QSIM_OP_2Q) o.q1 = tgt[o.q1];
+    }
+    bool move = false;
+    for (int p = 0; p < n; ++p) {
+        in_map[p] = tgt[st->perm_map[p]];
+        move |= in_map[p] != p;
+        after.map[tgt[p]] = p;
+    }
+    for (int p = 0; p < n; ++p) after.pending |= after.map[p] != p;
+    if (plan_for(st, mapped.data(), nops, mats, 1, plan, move ? in_map : nullptr)) return -1;
     if (jit_for(st, plan, sizeof(R) == 8, 1, fns)) return -1;
-    if (fold) {
+    if (move) {
         if (!jit_mapped_first(plan, sizeof(R) == 8, 1) || fns.empty() || !fns[0])
             return fail("internal: folded layout change without its kernel");
         if (ensure_tmp(st)) return -1;
-        st->perm_pending = false;
     }
     return 0;
+}
+
+void set_layout(qsim_state *st, const LayoutAfter &after)
+{
+    st->perm_pending = after.pending;
+    std::memcpy(st->perm_map, after.map, sizeof(int32_t) * st->n);
 }
 
 template <typename R>
@@ -416,12 +505,15 @@ int run_chunked(qsim_state *st, const qsim_op *ops, int nops, const void *mats, 
     (void)mat_count;
     Plan plan;
     std::vector<hipFunction_t> fns;
-    if (plan_and_compile<R>(st, ops, nops, mats, plan, fns)) return -1;
+    LayoutAfter after;
+    if (plan_and_compile<R>(st, ops, nops, mats, plan, fns, after)) return -1;
     DescImage img;
     img.build<R>(plan);
     void *dev = nullptr;
     if (st->up.upload(img.bytes.data(), img.bytes.size(), st->stream, &dev)) return -1;
-    return enqueue_passes<R>(st, plan, img, static_cast<char *>(dev), nullptr, nullptr, &fns);
+    if (enqueue_passes<R>(st, plan, img, static_cast<char *>(dev), nullptr, nullptr, &fns)) return -1;
+    set_layout(st, after);
+    return 0;
 }
 
 template <typename R>
@@ -441,34 +533,38 @@ int run_graphed(qsim_state *st, const qsim_op *ops, int nops, const void *mats, 
     put(&st->cmin, sizeof(int));
     put(&st->jit, sizeof(bool));
     put(&st->fuse, sizeof(bool));
-    const bool fold = can_fold(st);
-    put(&fold, sizeof(bool));
-    if (fold) put(st->perm_map, sizeof(int32_t) * st->n);
+    const bool keep = can_fold(st);
+    put(&keep, sizeof(bool));
+    if (keep) put(st->perm_map, sizeof(int32_t) * st->n);
     put(ops, sizeof(qsim_op) * nops);
     put(mats, sizeof(cx<R>) * mat_count);
     for (auto it = st->graphs.begin(); it != st->graphs.end(); ++it) {
         if (it->key == key) {
             st->graphs.splice(st->graphs.begin(), st->graphs, it);
             HIP_TRY(hipGraphLaunch(st->graphs.front().exec, st->stream));
-            if (st->graphs.front().swaps) {  // replayed the folded layout change
+            const GraphEntry &g = st->graphs.front();
+            if (g.swaps) {  // replayed a folded layout change
                 std::swap(st->psi, st->tmp);
-                st->perm_pending = false;
                 st->counters[6]++;
             }
+            st->perm_pending = g.after_pending;
+            std::memcpy(st->perm_map, g.after_map, sizeof(int32_t) * st->n);
             st->counters[1]++;
             st->counters[3] += 0;
             return 0;
         }
     }
-    if (fold && ensure_tmp(st)) return -1;
     Plan plan;
     std::vector<hipFunction_t> fns;
-    if (plan_and_compile<R>(st, ops, nops, mats, plan, fns)) return -1;
+    LayoutAfter after;
+    if (plan_and_compile<R>(st, ops, nops, mats, plan, fns, after)) return -1;
     DescImage img;
     img.build<R>(plan);
     GraphEntry e;
     e.key.swap(key);
     e.swaps = !plan.in_map.empty();
+    e.after_pending = after.pending;
+    std::memcpy(e.after_map, after.map, sizeof(int32_t) * st->n);
     HIP_TRY(hipMalloc(&e.desc, img.bytes.size()));
     HIP_TRY(hipMemcpyAsync(e.desc, img.bytes.data(), img.bytes.size(), hipMemcpyHostToDevice, st->stream));
     HIP_TRY(hipStreamSynchronize(st->stream));
@@ -495,6 +591,7 @@ int run_graphed(qsim_state *st, const qsim_op *ops, int nops, const void *mats, 
     st->graphs.push_front(std::move(e));
     HIP_TRY(hipGraphLaunch(st->graphs.front().exec, st->stream));
     st->counters[1]++;
+    set_layout(st, after);
     while (st->graphs.size() > kMaxGraphs) {
         HIP_TRY(hipStreamSynchronize(st->stream));
         GraphEntry &old = st->graphs.back();

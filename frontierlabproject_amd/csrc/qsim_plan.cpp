@@ -44,6 +44,103 @@ uint64_t fill_low(uint64_t set, int width, int target)
     return set;
 }
 
+// Ops of `masks` (qubit masks in program order) that one pass over chunk bits
+// M runs: an op runs when its qubits lie in M and no earlier op left behind
+// shares one of them.  Clears run entries of `live`; returns how many ran.
+int run_ops(const std::vector<uint64_t> &masks, std::vector<char> &live, uint64_t M)
+{
+    uint64_t blocked = 0;
+    int ran = 0;
+    for (size_t i = 0; i < masks.size(); ++i) {
+        if (!live[i]) continue;
+        const uint64_t m = masks[i];
+        if ((m & blocked) || (m & ~M)) {
+            blocked |= m;
+            continue;
+        }
+        live[i] = 0;
+        ++ran;
+    }
+    return ran;
+}
+
+// Passes the in-order rule needs for the live ops: each pass takes ops in
+// program order while the chunk (the `low` bits plus the ops' qubits) has at
+// most K bits.
+int greedy_passes(const std::vector<uint64_t> &masks, std::vector<char> live, uint64_t low, int K)
+{
+    int passes = 0;
+    for (;;) {
+        uint64_t P = low, blocked = 0;
+        bool any = false;
+        for (size_t i = 0; i < masks.size(); ++i) {
+            if (!live[i]) continue;
+            any = true;
+            const uint64_t m = masks[i];
+            if ((m & blocked) || popc(P | m) > K) {
+                blocked |= m;
+                continue;
+            }
+            P |= m;
+            live[i] = 0;
+        }
+        if (!any) return passes;
+        ++passes;
+    }
+}
+
+constexpr double kSearchMax = 20000;  // chunk-bit sets tried per pass
+
+// Free chunk bits of one pass chosen by search when the candidates are few:
+// over every set of `free` qubits among those the remaining ops touch, the
+// one after which the in-order rule needs the fewest further passes (the
+// in-order rule alone can strand an op that opens a third pass), then the one
+// running the most ops now.  Returns 0 when the search does not apply (all
+// qubits fit, or too many sets).
+uint64_t search_chunk(const PlanOp *ops, const std::vector<int> &rem, uint64_t P0, int Kp, uint64_t low, int K)
+{
+    const int free = Kp - popc(P0);
+    std::vector<uint64_t> masks;
+    masks.reserve(rem.size());
+    uint64_t Q = 0;
+    for (int idx : rem) {
+        uint64_t m = 1ull << ops[idx].q0;
+        if (ops[idx].kind == OP_2Q) m |= 1ull << ops[idx].q1;
+        masks.push_back(m);
+        Q |= m;
+    }
+    Q &= ~P0;
+    const int q = popc(Q);
+    if (free <= 0 || q <= free) return 0;
+    double combos = 1;
+    for (int i = 0; i < free; ++i) combos = combos * (q - i) / (i + 1);
+    if (combos > kSearchMax) return 0;
+    int bit[64], nb = 0;
+    for (int b = 0; b < 64; ++b)
+        if ((Q >> b) & 1) bit[nb++] = b;
+    uint64_t best = 0;
+    int best_rest = 1 << 30, best_ran = -1;
+    std::vector<char> live(masks.size());
+    for (uint64_t sel = (1ull << free) - 1; sel < (1ull << q);) {
+        uint64_t M = P0;
+        for (int i = 0; i < q; ++i)
+            if ((sel >> i) & 1) M |= 1ull << bit[i];
+        std::fill(live.begin(), live.end(), 1);
+        const int ran = run_ops(masks, live, M);
+        if (ran > 0) {
+            const int rest = greedy_passes(masks, live, low, K);
+            if (rest < best_rest || (rest == best_rest && ran > best_ran)) {
+                best = M;
+                best_rest = rest;
+                best_ran = ran;
+            }
+        }
+        const uint64_t c = sel & (~sel + 1), r = sel + c;  // next set of the same size
+        sel = (((r ^ sel) >> 2) / c) | r;
+    }
+    return best;
+}
+
 void make_pass(int n, int K, uint64_t P, PassDev &pd, int *loc)
 {
     pd = PassDev{};
@@ -384,6 +481,8 @@ bool plan_passes(int n, const PlanOp *ops, int nops, const void *mats, int elem_
             if (Kp < K) Kp = K;
         }
         first = false;
+        const uint64_t fixed = any_probe ? 0 : search_chunk(ops, rem, P, Kp, low, K);
+        if (fixed) P = fixed;
         int nprobe = 0;
         std::vector<int> chosen, left;
         for (int idx : rem) {
@@ -391,7 +490,7 @@ bool plan_passes(int n, const PlanOp *ops, int nops, const void *mats, int elem_
             uint64_t m = 1ull << op.q0;
             if (op.kind == OP_2Q) m |= 1ull << op.q1;
             const bool probe = op.kind == OP_PROBE;
-            if ((m & blocked) || popc(P | m) > Kp || (probe && nprobe >= max_probes)) {
+            if ((m & blocked) || popc(P | m) > Kp || (fixed && (m & ~fixed)) || (probe && nprobe >= max_probes)) {
                 blocked |= m;
                 left.push_back(idx);
                 continue;

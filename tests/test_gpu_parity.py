@@ -497,10 +497,11 @@ def _layouts(n, rng, k):
 @pytest.mark.parametrize("jit", [True, False])
 @pytest.mark.parametrize("mode", ["apply_tile_cooperative", "apply_tile_graphed", "apply_tile_dev"])
 def test_lazy_permutations_match_oracle(hip, dtype, jit, mode):
-    """permute() is applied lazily: folded into the next op list's first chunk
-    pass (complex64, specialised kernels) or run as a permutation pass.  Two
-    layout changes in a row compose; the result equals immediate permutation
-    followed by the gates, and a second identical run (graph replay) repeats it."""
+    """permute() is applied lazily.  complex64 chunk passes with specialised
+    kernels run on the stored bit order (op qubits mapped through it); other
+    paths apply the layout change as a permutation pass first.  Two layout
+    changes in a row compose; the result equals immediate permutation followed
+    by the gates, and a second identical run (graph replay) repeats it."""
     n = 14
     rng = np.random.default_rng(21)
     L = _layouts(n, rng, 4)
@@ -528,9 +529,56 @@ def test_lazy_permutations_match_oracle(hip, dtype, jit, mode):
         sv.permute(L[3], L[4])
         _check(ref, sv.to_numpy(), dtype)
     c = sv.counters()
-    folds = dtype == np.complex64 and jit and mode != "apply_tile_dev"
-    assert (c["folded_permutations"] == 6) if folds else c["folded_permutations"] == 0
-    if folds and mode == "apply_tile_graphed":
+    keep = dtype == np.complex64 and jit and mode != "apply_tile_dev"
+    # 14 qubits all lie below the low-region bound: a kept stored order moves
+    # no data before the read (one permutation pass per run, in to_numpy);
+    # otherwise each op list applies the pending change first (three per run)
+    assert c["folded_permutations"] == 0
+    assert c["permutations"] == (3 if keep else 9)
+    if keep and mode == "apply_tile_graphed":
+        assert c["graph_launches"] > c["graph_captures"]
+
+
+@pytest.mark.parametrize("mode", ["apply_tile_cooperative", "apply_tile_graphed"])
+def test_stored_layout_moves_op_qubits_low(hip, mode):
+    """complex64 with specialised kernels: an op list whose qubits are stored
+    at or above bit 18 trades them into the low bits inside its first chunk
+    pass (a folded layout change); the stored order then stays until a
+    reader.  Results equal the immediately permuted oracle on every run,
+    graph replays included."""
+    n = 20
+    rng = np.random.default_rng(33)
+    L = _layouts(n, rng, 3)
+    m1 = hip.permute_map(L[0], L[1])
+    hi = [m1[18], m1[19]]  # logical bits stored at 18 and 19 after the first change
+    rest = [int(q) for q in rng.permutation(n) if q not in hi]
+    T1 = hi + rest[:10]
+    T2 = [int(q) for q in rng.permutation(n)[:14]]
+
+    def ops_on(T, depth, seed):
+        return [(o[0], *[T[q] for q in o[1:-1]], o[-1]) for o in _tuples(C.random_circuit(len(T), depth, seed))]
+
+    ops1, ops2 = ops_on(T1, 3, 41), ops_on(T2, 3, 42)
+    psi0 = C.random_state(n, 12)
+    ref = P.permute(psi0, L[0], L[1])
+    for o in ops1:
+        ref = dense.apply(ref, o[-1], list(o[1:-1]), n)
+    ref = P.permute(ref, L[1], L[2])
+    for o in ops2:
+        ref = dense.apply(ref, o[-1], list(o[1:-1]), n)
+    ref = P.permute(ref, L[2], L[3])
+    sv = hip.StatevectorF32(n)
+    for rep in range(3):
+        sv.from_numpy(psi0)
+        sv.permute(L[0], L[1])
+        getattr(sv, mode)(ops1)
+        sv.permute(L[1], L[2])
+        getattr(sv, mode)(ops2)
+        sv.permute(L[2], L[3])
+        _check(ref, sv.to_numpy(), np.complex64)
+    c = sv.counters()
+    assert c["folded_permutations"] >= 3
+    if mode == "apply_tile_graphed":
         assert c["graph_launches"] > c["graph_captures"]
 
 
