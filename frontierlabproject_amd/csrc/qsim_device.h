@@ -310,10 +310,6 @@ __device__ __forceinline__ void slot2(cx<R> *a, cx<R> *l, int kind, const cx<R> 
 // NV = 1: forward on psi.  NV = 2: reverse sweep on (psi, lam) with gradient
 // probes Im<lam|G|psi> reduced per block.
 // ----------------------------------------------------------------------------
-// LDS slot of chunk element e: bits 6..10 are folded into bits 1..5, which
-// spreads the strided step gathers over banks and keeps (e, e^1) adjacent so
-// amplitude pairs move as one 16-byte access.
-__device__ __forceinline__ int swz(int e) { return e ^ (((e >> 6) & 31) << 1); }
 
 // Chunk <-> global copies.  Element e sits at base + rowoff[e >> cf] + (e & low).
 // complex64 with cf >= 1 moves two amplitudes per 16-byte access.  Each thread
@@ -347,6 +343,8 @@ __device__ __forceinline__ void gstore(V *p, V v)
         *p = v;
 }
 
+// TPB is a power of two, so element (v0 + j * TPB) + threadIdx.x splits into
+// disjoint bit fields and its slot is swz(uniform part) ^ swz(thread part).
 template <typename R, bool LOAD, bool NT = false, int TPB = kTPB>
 __device__ __forceinline__ void chunk_copy(cx<R> *__restrict__ g, cx<R> *__restrict__ lds, uint64_t base,
                                            const uint64_t *__restrict__ rowoff, int E, int cf)
@@ -354,6 +352,7 @@ __device__ __forceinline__ void chunk_copy(cx<R> *__restrict__ g, cx<R> *__restr
     const int lowmask = (1 << cf) - 1;
     if (sizeof(cx<R>) == 8 && cf >= 1) {
         const int nv = E >> 1;
+        const int st = swz((int)threadIdx.x << 1);
         for (int v0 = 0; v0 < nv; v0 += TPB * kBatch) {
             v4f buf[kBatch];
             if (LOAD) {
@@ -366,13 +365,13 @@ __device__ __forceinline__ void chunk_copy(cx<R> *__restrict__ g, cx<R> *__restr
 #pragma unroll
                 for (int j = 0; j < kBatch; ++j) {
                     const int v = v0 + j * TPB + threadIdx.x;
-                    if (v < nv) *reinterpret_cast<v4f *>(lds + swz(v << 1)) = buf[j];
+                    if (v < nv) *reinterpret_cast<v4f *>(lds + (st ^ swz((v0 + j * TPB) << 1))) = buf[j];
                 }
             } else {
 #pragma unroll
                 for (int j = 0; j < kBatch; ++j) {
                     const int v = v0 + j * TPB + threadIdx.x;
-                    if (v < nv) buf[j] = *reinterpret_cast<const v4f *>(lds + swz(v << 1));
+                    if (v < nv) buf[j] = *reinterpret_cast<const v4f *>(lds + (st ^ swz((v0 + j * TPB) << 1)));
                 }
 #pragma unroll
                 for (int j = 0; j < kBatch; ++j) {
@@ -387,6 +386,7 @@ __device__ __forceinline__ void chunk_copy(cx<R> *__restrict__ g, cx<R> *__restr
         const raw *gr = reinterpret_cast<const raw *>(g);
         raw *gw = reinterpret_cast<raw *>(g);
         raw *lr = reinterpret_cast<raw *>(lds);
+        const int st = swz((int)threadIdx.x);
         for (int e0 = 0; e0 < E; e0 += TPB * kBatch) {
             raw buf[kBatch];
             if (LOAD) {
@@ -398,13 +398,13 @@ __device__ __forceinline__ void chunk_copy(cx<R> *__restrict__ g, cx<R> *__restr
 #pragma unroll
                 for (int j = 0; j < kBatch; ++j) {
                     const int e = e0 + j * TPB + threadIdx.x;
-                    if (e < E) lr[swz(e)] = buf[j];
+                    if (e < E) lr[st ^ swz(e0 + j * TPB)] = buf[j];
                 }
             } else {
 #pragma unroll
                 for (int j = 0; j < kBatch; ++j) {
                     const int e = e0 + j * TPB + threadIdx.x;
-                    if (e < E) buf[j] = lr[swz(e)];
+                    if (e < E) buf[j] = lr[st ^ swz(e0 + j * TPB)];
                 }
 #pragma unroll
                 for (int j = 0; j < kBatch; ++j) {
@@ -436,6 +436,8 @@ __device__ __forceinline__ void chunk_copy_in_mapped(const cx<float> *__restrict
                                                      int cf, Sigma sigma, int jb0)
 {
     const int lowmask = (1 << cf) - 1, nv = E >> 1;
+    // sigma and swz are linear: the slot of a read index splits like its bits
+    const int st = swz(sigma((int)threadIdx.x << 1)), s1 = swz(1 << jb0);
     for (int v0 = 0; v0 < nv; v0 += TPB * kBatch) {
         v4f buf[kBatch];
 #pragma unroll
@@ -448,9 +450,9 @@ __device__ __forceinline__ void chunk_copy_in_mapped(const cx<float> *__restrict
         for (int j = 0; j < kBatch; ++j) {
             const int v = v0 + j * TPB + threadIdx.x;
             if (v < nv) {
-                const int e = sigma(v << 1);
-                lds[swz(e)] = cx<float>{buf[j].x, buf[j].y};
-                lds[swz(e | (1 << jb0))] = cx<float>{buf[j].z, buf[j].w};
+                const int sl = st ^ swz(sigma((v0 + j * TPB) << 1));
+                lds[sl] = cx<float>{buf[j].x, buf[j].y};
+                lds[sl ^ s1] = cx<float>{buf[j].z, buf[j].w};
             }
         }
     }

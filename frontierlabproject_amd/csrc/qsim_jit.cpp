@@ -25,7 +25,7 @@ std::map<std::pair<int, std::string>, hipFunction_t> g_fns;  // (device, source)
 std::vector<hipModule_t> g_modules;                           // never unloaded
 std::atomic<long> g_compiled{0};
 
-int swz_host(int e) { return e ^ (((e >> 6) & 31) << 1); }
+int swz_host(int e) { return swz(e); }
 
 uint64_t fnv1a(const std::string &s)
 {
@@ -69,8 +69,13 @@ void emit_step(std::ostringstream &o, const StepDev &sd, int E, const char *T, i
           << ");\n";
     else
         o << "    const uint32_t g32 = (uint32_t)gi;\n";
+    // with chunk bit 0 among the register bits the base slot is even (swz
+    // keeps bit 0); saying so lets the compiler pair slots sb, sb ^ 1 into
+    // 16-byte LDS accesses
+    bool bit0 = false;
+    for (int q = 0; q < 4; ++q) bit0 |= sd.bits[q] == 0;
     o << "    const int sb = swz((int)ins0(ins0(ins0(ins0(g32, " << sd.sbits[0] << "), " << sd.sbits[1] << "), "
-      << sd.sbits[2] << "), " << sd.sbits[3] << "));\n";
+      << sd.sbits[2] << "), " << sd.sbits[3] << "))" << (bit0 ? " & ~1" : "") << ";\n";
     // vectors: a (psi, in c0) and, with nv == 2, l (lam, in c1)
     const char *vec[2] = {"a", "l"}, *lds[2] = {"c0", "c1"};
     for (int v = 0; v < nv; ++v) {
