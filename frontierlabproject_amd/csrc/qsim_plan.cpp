@@ -90,14 +90,21 @@ int greedy_passes(const std::vector<uint64_t> &masks, std::vector<char> live, ui
 }
 
 constexpr double kSearchMax = 20000;  // chunk-bit sets tried per pass
+// Source bits from here up lie 2 MiB of complex64 amplitudes apart or more: a
+// chunk gathering over them reads rows from separate address-translation
+// pages.
+constexpr int kFarBit = 18;
 
 // Free chunk bits of one pass chosen by search when the candidates are few:
 // over every set of `free` qubits among those the remaining ops touch, the
 // one after which the in-order rule needs the fewest further passes (the
 // in-order rule alone can strand an op that opens a third pass), then the one
+// with the fewest bits of `far` (bits the pass reads from far-apart rows: for
+// a folded layout change, those stored high in the old order), then the one
 // running the most ops now.  Returns 0 when the search does not apply (all
 // qubits fit, or too many sets).
-uint64_t search_chunk(const PlanOp *ops, const std::vector<int> &rem, uint64_t P0, int Kp, uint64_t low, int K)
+uint64_t search_chunk(const PlanOp *ops, const std::vector<int> &rem, uint64_t P0, int Kp, uint64_t low, int K,
+                      uint64_t far)
 {
     const int free = Kp - popc(P0);
     std::vector<uint64_t> masks;
@@ -119,7 +126,7 @@ uint64_t search_chunk(const PlanOp *ops, const std::vector<int> &rem, uint64_t P
     for (int b = 0; b < 64; ++b)
         if ((Q >> b) & 1) bit[nb++] = b;
     uint64_t best = 0;
-    int best_rest = 1 << 30, best_ran = -1;
+    int best_rest = 1 << 30, best_far = 1 << 30, best_ran = -1;
     std::vector<char> live(masks.size());
     for (uint64_t sel = (1ull << free) - 1; sel < (1ull << q);) {
         uint64_t M = P0;
@@ -129,9 +136,11 @@ uint64_t search_chunk(const PlanOp *ops, const std::vector<int> &rem, uint64_t P
         const int ran = run_ops(masks, live, M);
         if (ran > 0) {
             const int rest = greedy_passes(masks, live, low, K);
-            if (rest < best_rest || (rest == best_rest && ran > best_ran)) {
+            const int nfar = popc(M & far);
+            if (rest < best_rest || (rest == best_rest && (nfar < best_far || (nfar == best_far && ran > best_ran)))) {
                 best = M;
                 best_rest = rest;
+                best_far = nfar;
                 best_ran = ran;
             }
         }
@@ -481,7 +490,10 @@ bool plan_passes(int n, const PlanOp *ops, int nops, const void *mats, int elem_
             if (Kp < K) Kp = K;
         }
         first = false;
-        const uint64_t fixed = any_probe ? 0 : search_chunk(ops, rem, P, Kp, low, K);
+        uint64_t far = 0;  // chunk bits this pass would gather from far-apart rows
+        if (in_map && out.passes.empty())
+            for (int p = kFarBit; p < n; ++p) far |= 1ull << in_map[p];
+        const uint64_t fixed = any_probe ? 0 : search_chunk(ops, rem, P, Kp, low, K, far);
         if (fixed) P = fixed;
         int nprobe = 0;
         std::vector<int> chosen, left;
