@@ -393,18 +393,19 @@ int jit_for(qsim_state *st, const Plan &plan, bool dbl, int nv, std::vector<hipF
 }
 
 // A stored order that differs from the logical one is kept across op lists
-// when their chunk passes run specialised complex64 kernels (a layout change
-// then folds into a first pass); otherwise it is applied first.
-bool can_fold(const qsim_state *st) { return st->perm_pending && st->jit && st->dtype == QSIM_C64; }
+// when their chunk passes run specialised kernels (a layout change then folds
+// into a first pass); otherwise it is applied first.
+bool can_fold(const qsim_state *st) { return st->perm_pending && st->jit; }
 
-// Stored bits below kLowRegion span one 2 MiB window of complex64 amplitudes:
-// chunk passes over them gather and scatter within few address-translation
-// pages, while chunks over higher bits touch up to one page per row.
-constexpr int kLowRegion = 18;
+// Stored bits below low_region() span one 2 MiB window of amplitudes (2^18
+// complex64, 2^17 complex128): chunk passes over them gather and scatter
+// within few address-translation pages, while chunks over higher bits touch
+// up to one page per row.
+int low_region(const qsim_state *st) { return st->dtype == QSIM_C64 ? 18 : 17; }
 
 // Stored position tgt[q] for each logical bit q while running `ops` on a
 // stored order that differs from the logical one.  The op qubits should sit
-// below r = max(#op qubits, kLowRegion).  Each op qubit stored at or above r
+// below r = max(#op qubits, low_region()).  Each op qubit stored at or above r
 // takes the slot of a non-op qubit below r (the row bits 0..cmin-1 first, so
 // every pass's reserved low bits hold op qubits); that qubit moves to the
 // vacated position, except that one leaving a row bit takes the lowest
@@ -422,7 +423,7 @@ void choose_layout(const qsim_state *st, const qsim_op *ops, int nops, int32_t *
         T |= 1ull << ops[i].q0;
         if (ops[i].kind == QSIM_OP_2Q) T |= 1ull << ops[i].q1;
     }
-    const int r = std::min(n, std::max(__builtin_popcountll(T), kLowRegion));
+    const int r = std::min(n, std::max(__builtin_popcountll(T), low_region(st)));
     std::vector<int> slots, in;  // ascending stored positions
     for (int p = 0; p < r; ++p)
         if (!((T >> at[p]) & 1)) slots.push_back(p);
@@ -1160,7 +1161,7 @@ int qsim_jit_compile_check(int num_qubits, int dtype, const qsim_op *ops, int no
     std::string err;
     if (!plan_passes(num_qubits, pops.data(), nops, mats, (int)amp_bytes(dtype), K, cmin,
                      nv == 2 ? kMaxProbes : 1, plan, err, fuse != 0 && nv == 1,
-                     nv == 1 && dtype == QSIM_C64 ? in_map : nullptr))
+                     nv == 1 ? in_map : nullptr))
         return fail(err);
     if (!jit_compile_only(plan, dtype == QSIM_C128, nv, "gfx950", err))
         return fail("run-time kernel compile: " + err);

@@ -458,6 +458,34 @@ __device__ __forceinline__ void chunk_copy_in_mapped(const cx<float> *__restrict
     }
 }
 
+// complex128 form: one 16-byte amplitude per read, so any source order works;
+// read index r sits at base + rowoff[r >> cf] + (r & (2^cf - 1)) and lands at
+// chunk index sigma(r).
+template <int TPB, typename Sigma>
+__device__ __forceinline__ void chunk_copy_in_mapped_d(const cx<double> *__restrict__ g, cx<double> *__restrict__ lds,
+                                                       uint64_t base, const uint64_t *__restrict__ rowoff, int E,
+                                                       int cf, Sigma sigma)
+{
+    typedef typename RawAmp<double>::type raw;
+    const raw *gr = reinterpret_cast<const raw *>(g);
+    raw *lr = reinterpret_cast<raw *>(lds);
+    const int lowmask = (1 << cf) - 1;
+    const int st = swz(sigma((int)threadIdx.x));
+    for (int r0 = 0; r0 < E; r0 += TPB * kBatch) {
+        raw buf[kBatch];
+#pragma unroll
+        for (int j = 0; j < kBatch; ++j) {
+            const int r = r0 + j * TPB + threadIdx.x;
+            if (r < E) buf[j] = gload<true>(gr + base + rowoff[r >> cf] + (r & lowmask));
+        }
+#pragma unroll
+        for (int j = 0; j < kBatch; ++j) {
+            const int r = r0 + j * TPB + threadIdx.x;
+            if (r < E) lr[st ^ swz(sigma(r0 + j * TPB))] = buf[j];
+        }
+    }
+}
+
 // The register steps of one pass over the chunk staged in LDS (c0, and c1 for
 // the second vector).  Ends with a barrier after every step.
 template <typename R, int NV>

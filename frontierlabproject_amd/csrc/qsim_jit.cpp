@@ -252,9 +252,10 @@ std::string bit_moves(const char *x, const int *dst, int nbits, const char *ty)
 
 // First pass of a plan with a folded layout change: reads `psi` in the
 // source layout, writes `out` in the pass layout (complex64, one vector).
-std::string mapped_pass_source(const Plan &plan, const std::string &name)
+std::string mapped_pass_source(const Plan &plan, bool dbl, const std::string &name)
 {
     const PassDev &pd = plan.passes[0];
+    const char *T = dbl ? "double" : "float";
     const int n = (int)plan.in_map.size();
     const int K = pd.K, CF = pd.cfast, E = 1 << K, NROW = 1 << (K - CF);
     const int TPB = jit_block_threads(K, 1);
@@ -272,10 +273,10 @@ std::string mapped_pass_source(const Plan &plan, const std::string &name)
     int comp_old[64];
     for (int m = 0; m < pd.ncomp; ++m) comp_old[m] = inv[pd.comp[m]];
     std::ostringstream o;
-    o << "extern \"C\" __global__ __launch_bounds__(" << TPB << ") __attribute__((amdgpu_waves_per_eu(4)))\nvoid "
-      << name << "(const cx<float> *__restrict__ psi, cx<float> *__restrict__ out, "
-      << "const cx<float> *__restrict__ mats)\n{\n";
-    o << "  __shared__ __align__(16) cx<float> c0[" << E << "];\n";
+    o << "extern \"C\" __global__ __launch_bounds__(" << TPB << ")"
+      << (dbl ? "" : " __attribute__((amdgpu_waves_per_eu(4)))") << "\nvoid " << name << "(const cx<" << T
+      << "> *__restrict__ psi, cx<" << T << "> *__restrict__ out, const cx<" << T << "> *__restrict__ mats)\n{\n";
+    o << "  __shared__ __align__(16) cx<" << T << "> c0[" << E << "];\n";
     o << "  __shared__ uint64_t rowoff[" << NROW << "];\n  __shared__ uint64_t rowin[" << NROWIN << "];\n";
     o << "  for (int t = threadIdx.x; t < " << NROW << "; t += " << TPB << ")\n    rowoff[t] = 0ull"
       << bit_moves("t", pd.pos + CF, K - CF, "uint64_t") << ";\n";
@@ -286,11 +287,15 @@ std::string mapped_pass_source(const Plan &plan, const std::string &name)
     o << "  const uint64_t base_in = 0ull" << bit_moves("c", comp_old, pd.ncomp, "uint64_t") << ";\n";
     o << "  auto sigma = [](int r) -> int { return 0" << bit_moves("r", J, K, "unsigned") << "; };\n";
     o << "  __syncthreads();\n";
-    o << "  chunk_copy_in_mapped<" << TPB << ">(psi, c0, base_in, rowin, " << E << ", " << CFR << ", sigma, " << J[0]
-      << ");\n";
+    if (dbl)
+        o << "  chunk_copy_in_mapped_d<" << TPB << ">(psi, c0, base_in, rowin, " << E << ", " << CFR << ", sigma);\n";
+    else
+        o << "  chunk_copy_in_mapped<" << TPB << ">(psi, c0, base_in, rowin, " << E << ", " << CFR << ", sigma, "
+          << J[0] << ");\n";
     o << "  __syncthreads();\n";
-    for (int s = 0; s < pd.nsteps; ++s) emit_step(o, plan.steps[pd.step_off + s], E, "float", 1, TPB);
-    o << "  chunk_copy<float, false, true, " << TPB << ">(out, c0, base, rowoff, " << E << ", " << CF << ");\n}\n\n";
+    for (int s = 0; s < pd.nsteps; ++s) emit_step(o, plan.steps[pd.step_off + s], E, T, 1, TPB);
+    o << "  chunk_copy<" << T << ", false, true, " << TPB << ">(out, c0, base, rowoff, " << E << ", " << CF
+      << ");\n}\n\n";
     return o.str();
 }
 
@@ -298,7 +303,8 @@ std::string mapped_pass_source(const Plan &plan, const std::string &name)
 
 bool jit_mapped_first(const Plan &plan, bool dbl, int nv)
 {
-    if (plan.in_map.empty() || dbl || nv != 1 || plan.passes.empty()) return false;
+    if (plan.in_map.empty() || nv != 1 || plan.passes.empty()) return false;
+    if (dbl) return plan.passes[0].K >= 2;  // one amplitude per 16-byte read
     const PassDev &pd = plan.passes[0];
     int inv[64];
     for (size_t q = 0; q < plan.in_map.size(); ++q) inv[plan.in_map[q]] = (int)q;
@@ -309,7 +315,7 @@ bool jit_mapped_first(const Plan &plan, bool dbl, int nv)
 
 std::string jit_pass_source(const Plan &plan, size_t p, bool dbl, int nv, const std::string &name)
 {
-    if (p == 0 && jit_mapped_first(plan, dbl, nv)) return mapped_pass_source(plan, name);
+    if (p == 0 && jit_mapped_first(plan, dbl, nv)) return mapped_pass_source(plan, dbl, name);
     const PassDev &pd = plan.passes[p];
     const char *T = dbl ? "double" : "float";
     const int K = pd.K, CF = pd.cfast, E = 1 << K, NROW = 1 << (K - CF);

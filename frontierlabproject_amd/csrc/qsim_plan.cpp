@@ -90,10 +90,10 @@ int greedy_passes(const std::vector<uint64_t> &masks, std::vector<char> live, ui
 }
 
 constexpr double kSearchMax = 20000;  // chunk-bit sets tried per pass
-// Source bits from here up lie 2 MiB of complex64 amplitudes apart or more: a
-// chunk gathering over them reads rows from separate address-translation
-// pages.
-constexpr int kFarBit = 18;
+// Source bits from log2(2 MiB / amplitude size) up (18 for complex64, 17
+// for complex128) lie 2 MiB apart or more: a chunk gathering over them reads
+// rows from separate address-translation pages.
+inline int far_bit(int elem_bytes) { return elem_bytes == 8 ? 18 : 17; }
 
 // Free chunk bits of one pass chosen by search when the candidates are few:
 // over every set of `free` qubits among those the remaining ops touch, the
@@ -492,7 +492,7 @@ bool plan_passes(int n, const PlanOp *ops, int nops, const void *mats, int elem_
         first = false;
         uint64_t far = 0;  // chunk bits this pass would gather from far-apart rows
         if (in_map && out.passes.empty())
-            for (int p = kFarBit; p < n; ++p) far |= 1ull << in_map[p];
+            for (int p = far_bit(elem_bytes); p < n; ++p) far |= 1ull << in_map[p];
         const uint64_t fixed = any_probe ? 0 : search_chunk(ops, rem, P, Kp, low, K, far);
         if (fixed) P = fixed;
         int nprobe = 0;

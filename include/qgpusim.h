@@ -91,13 +91,16 @@ int qsim_permute_map(int num_qubits, const int32_t *layout_old, const int32_t *l
                      int32_t *map_out);
 
 /* permute(layout_old, layout_new) (bindings.cpp:434-483), with the same map
- * and errors.  The layout change is recorded (composed with one still
- * pending) and applied lazily: the next qsim_apply_ops in a chunked mode on a
- * complex64 state with specialised kernels folds it into its first chunk
- * pass, which reads the old layout and writes the new one into the second
- * buffer; any other use of the state first applies it as one LDS-tiled
- * gather/scatter pass into the second buffer, after which the buffers swap.
- * Results are those of an immediate permutation. */
+ * and errors.  No data moves: the layout change is composed into the state's
+ * stored bit order (which may differ from the logical layout).  An op list
+ * in a chunked mode with specialised kernels runs on the stored order, its
+ * qubits mapped through it; when some of them are stored high, its first
+ * chunk pass reads the old stored order and writes one where they sit low
+ * (only they and the qubits they displace move) into the second buffer.  Any
+ * other use of the state (reads, reductions, single gates, the sequential
+ * mode, kernels compiled ahead of time) first restores the logical layout
+ * with one LDS-tiled gather/scatter pass into the second buffer.  Results are
+ * those of an immediate permutation. */
 int qsim_permute(qsim_state *st, const int32_t *layout_old, const int32_t *layout_new, int num_qubits);
 
 /* Host only: the integer checks of parse_tile_ops (bindings.cpp:196-216). */
@@ -169,7 +172,7 @@ int qsim_set_jit(qsim_state *st, int enable);
 /* Host only, no reference counterpart: plan `ops` as qsim_apply_ops would (as
  * the adjoint's reverse sweep would, on psi and lam, when `ops` hold
  * QSIM_OP_PROBE entries; with a pending layout change in_map[p] = new
- * position of bit p, complex64 forward only, or NULL) and compile, without
+ * position of bit p, forward only, or NULL) and compile, without
  * loading, every specialised pass kernel for gfx950.  Returns the number of
  * passes, or -1 with the compiler log. */
 int qsim_jit_compile_check(int num_qubits, int dtype, const qsim_op *ops, int nops,

@@ -497,9 +497,9 @@ def _layouts(n, rng, k):
 @pytest.mark.parametrize("jit", [True, False])
 @pytest.mark.parametrize("mode", ["apply_tile_cooperative", "apply_tile_graphed", "apply_tile_dev"])
 def test_lazy_permutations_match_oracle(hip, dtype, jit, mode):
-    """permute() is applied lazily.  complex64 chunk passes with specialised
-    kernels run on the stored bit order (op qubits mapped through it); other
-    paths apply the layout change as a permutation pass first.  Two layout
+    """permute() is applied lazily.  Chunk passes with specialised kernels run
+    on the stored bit order (op qubits mapped through it); other paths apply
+    the layout change as a permutation pass first.  Two layout
     changes in a row compose; the result equals immediate permutation followed
     by the gates, and a second identical run (graph replay) repeats it."""
     n = 14
@@ -529,7 +529,7 @@ def test_lazy_permutations_match_oracle(hip, dtype, jit, mode):
         sv.permute(L[3], L[4])
         _check(ref, sv.to_numpy(), dtype)
     c = sv.counters()
-    keep = dtype == np.complex64 and jit and mode != "apply_tile_dev"
+    keep = jit and mode != "apply_tile_dev"
     # 14 qubits all lie below the low-region bound: a kept stored order moves
     # no data before the read (one permutation pass per run, in to_numpy);
     # otherwise each op list applies the pending change first (three per run)
@@ -539,13 +539,14 @@ def test_lazy_permutations_match_oracle(hip, dtype, jit, mode):
         assert c["graph_launches"] > c["graph_captures"]
 
 
+@pytest.mark.parametrize("dtype", [np.complex64, np.complex128])
 @pytest.mark.parametrize("mode", ["apply_tile_cooperative", "apply_tile_graphed"])
-def test_stored_layout_moves_op_qubits_low(hip, mode):
-    """complex64 with specialised kernels: an op list whose qubits are stored
-    at or above bit 18 trades them into the low bits inside its first chunk
-    pass (a folded layout change); the stored order then stays until a
-    reader.  Results equal the immediately permuted oracle on every run,
-    graph replays included."""
+def test_stored_layout_moves_op_qubits_low(hip, mode, dtype):
+    """Specialised kernels: an op list whose qubits are stored at or above the
+    low region (bit 18 for complex64, 17 for complex128) trades them into the
+    low bits inside its first chunk pass (a folded layout change); the stored
+    order then stays until a reader.  Results equal the immediately permuted
+    oracle on every run, graph replays included."""
     n = 20
     rng = np.random.default_rng(33)
     L = _layouts(n, rng, 3)
@@ -567,7 +568,7 @@ def test_stored_layout_moves_op_qubits_low(hip, mode):
     for o in ops2:
         ref = dense.apply(ref, o[-1], list(o[1:-1]), n)
     ref = P.permute(ref, L[2], L[3])
-    sv = hip.StatevectorF32(n)
+    sv = _sv(hip, n, dtype)
     for rep in range(3):
         sv.from_numpy(psi0)
         sv.permute(L[0], L[1])
@@ -575,7 +576,7 @@ def test_stored_layout_moves_op_qubits_low(hip, mode):
         sv.permute(L[1], L[2])
         getattr(sv, mode)(ops2)
         sv.permute(L[2], L[3])
-        _check(ref, sv.to_numpy(), np.complex64)
+        _check(ref, sv.to_numpy(), dtype)
     c = sv.counters()
     assert c["folded_permutations"] >= 3
     if mode == "apply_tile_graphed":

@@ -363,19 +363,21 @@ def test_batch_spec_and_generator_are_reproducible():
     assert [ci.operation.name for ci in qc.data] == [g[0] for g in stream]
 
 
+@pytest.mark.parametrize("dtype", [np.complex64, np.complex128])
 @pytest.mark.parametrize("seed", [1, 2, 3])
-def test_folded_layout_change_kernels_compile(seed):
+def test_folded_layout_change_kernels_compile(seed, dtype):
     """First passes that fold a pending layout change (read the old layout,
     write the new one) compile for gfx950, with and without gates."""
     from frontierlabproject_amd.backends.custom.native import qgpusim_hip as hip
     n = 16
+    K, cmin = (12, 4) if dtype == np.complex64 else (11, 3)
     rng = np.random.default_rng(seed)
     in_map = rng.permutation(n).astype(np.int32)
     ops = [("1q", int(q), G.h()) for q in rng.integers(0, n, 6)] + [("2q", 0, 5, G.cx())]
-    arr, arena = hip.pack_ops(ops, n, np.complex64)
-    assert hip.jit_compile_check(n, arr, len(ops), arena, 12, 4, in_map=in_map) > 0
-    arr0, arena0 = hip.pack_ops([], n, np.complex64)
-    assert hip.jit_compile_check(n, arr0, 0, arena0, 12, 4, in_map=in_map) == 1
+    arr, arena = hip.pack_ops(ops, n, dtype)
+    assert hip.jit_compile_check(n, arr, len(ops), arena, K, cmin, in_map=in_map) > 0
+    arr0, arena0 = hip.pack_ops([], n, dtype)
+    assert hip.jit_compile_check(n, arr0, 0, arena0, K, cmin, in_map=in_map) == 1
 
 
 def test_statevector_overlap_and_tvd():
