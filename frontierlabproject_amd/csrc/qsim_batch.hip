@@ -44,15 +44,18 @@ __device__ double run_resident(const BatchCirc &C, const BatchArgs &A, cx<float>
                                double *pacc, double *sh)
 {
     const int E = 1 << C.n;
-    for (int e = threadIdx.x; e < E; e += kTPB) c0[swz(e)] = cx<float>{e == 0 ? 1.f : 0.f, 0.f};
+    // slots split as swz(e - threadIdx.x) ^ swz(threadIdx.x): kTPB is a power of two
+    const int st = swz((int)threadIdx.x);
+    for (int e = threadIdx.x; e < E; e += kTPB) c0[swz(e - threadIdx.x) ^ st] = cx<float>{e == 0 ? 1.f : 0.f, 0.f};
     __syncthreads();
     for (int p = 0; p < C.nfwd; ++p) run_steps<float, 1>(c0, c1, A.passes + C.fwd0 + p, A.steps, A.mats, pacc);
     NSum acc;
     for (int e = threadIdx.x; e < E; e += kTPB) {
-        const cx<float> v = c0[swz(e)];
+        const int sl = swz(e - threadIdx.x) ^ st;
+        const cx<float> v = c0[sl];
         const float sg = zsign((uint64_t)e, C.zmask);
         acc.add((double)sg * ((double)v.re * v.re + (double)v.im * v.im));
-        c1[swz(e)] = cx<float>{sg * v.re, sg * v.im};
+        c1[sl] = cx<float>{sg * v.re, sg * v.im};
     }
     const double ez = block_sum(acc.value(), sh);
     __syncthreads();
@@ -85,8 +88,9 @@ __device__ double run_streamed(const BatchCirc &C, const BatchArgs &A, cx<float>
         for (uint64_t c = 0; c < nchunks; ++c) {
             const uint64_t base = chunk_base(c, pd);
             if (p == 0) {  // the input is |0>: generate the chunk instead of reading it
+                const int st = swz((int)threadIdx.x);
                 for (int e = threadIdx.x; e < E; e += kTPB)
-                    c0[swz(e)] = cx<float>{(base == 0 && e == 0) ? 1.f : 0.f, 0.f};
+                    c0[swz(e - threadIdx.x) ^ st] = cx<float>{(base == 0 && e == 0) ? 1.f : 0.f, 0.f};
             } else {
                 chunk_copy<float, true>(psi, c0, base, rowoff, E, cf);
             }

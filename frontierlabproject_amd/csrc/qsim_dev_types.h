@@ -49,6 +49,8 @@ QSIM_HD inline constexpr int swz(int e)
         if ((e >> k) & 1) s ^= swz_pattern(k);
     return s;
 }
+// Slot of the single element 2^k.
+QSIM_HD inline constexpr int swz_bit(int k) { return (1 << k) ^ swz_pattern(k); }
 constexpr int kMaxComp = 50;   // bits outside a chunk
 #ifndef QSIM_ROUNDS
 #define QSIM_ROUNDS 4

@@ -505,11 +505,12 @@ __device__ __forceinline__ void run_steps(cx<R> *c0, cx<R> *c1, const PassDev *_
         // swz is XOR-linear and the group base and register offsets occupy
         // disjoint bits, so each element's LDS slot is swz(base) ^ swz(offset):
         // one XOR per element with a wave-uniform constant.
+        // (the 16 offsets are XOR combinations of the four register bits' slots)
         int moff[16];
+        const int b0 = swz_bit(r0), b1 = swz_bit(r1), b2 = swz_bit(r2), b3 = swz_bit(r3);
 #pragma unroll
         for (int m = 0; m < 16; ++m)
-            moff[m] = swz(((m & 1) << r0) | (((m >> 1) & 1) << r1) | (((m >> 2) & 1) << r2) |
-                          (((m >> 3) & 1) << r3));
+            moff[m] = ((m & 1) ? b0 : 0) ^ ((m & 2) ? b1 : 0) ^ ((m & 4) ? b2 : 0) ^ ((m & 8) ? b3 : 0);
         // wave-uniform trip count: probes reduce across all 64 lanes
         for (int gb = 0; gb < ngroups; gb += kTPB) {
             const int gi = gb + threadIdx.x;

@@ -596,12 +596,14 @@ __global__ __launch_bounds__(kTPB) void k_pauli_chunk(const cx<R> *__restrict__ 
     __syncthreads();
 
     NSum er, ei;
+    const int st = swz((int)threadIdx.x);  // slots split like e = (e - threadIdx.x) ^ threadIdx.x
     for (int e = threadIdx.x; e < E; e += kTPB) {
         const uint64_t addr = base + rowoff[e >> cf] + (e & lowmask);
+        const int se = swz(e - threadIdx.x) ^ st;
         double tr = 0.0, ti = 0.0;
         for (int k = 0; k < nterms; ++k) {
             const PauliDev T = terms[k];
-            const cx<R> v = c0[swz(e ^ T.xloc)];
+            const cx<R> v = c0[se ^ swz(T.xloc)];
             const uint64_t src = addr ^ T.xmask;
             // i^{|x&z|} * (-1)^{popc(src & z)}
             const int ph = (__popcll(T.xmask & T.zmask) + 2 * __popcll(src & T.zmask)) & 3;
@@ -624,7 +626,7 @@ __global__ __launch_bounds__(kTPB) void k_pauli_chunk(const cx<R> *__restrict__ 
             lam[addr] = o;
         }
         if (mode & PAULI_EXPECT) {
-            const cx<R> p = c0[swz(e)];
+            const cx<R> p = c0[se];
             const double pr = p.re, pi = p.im;
             er.add(pr * tr + pi * ti);
             ei.add(pr * ti - pi * tr);
