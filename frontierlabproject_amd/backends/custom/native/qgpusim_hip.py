@@ -340,8 +340,10 @@ class _StatevectorBase:
         return out[:int(shots)]
 
     def set_fusion(self, mode):
-        """0: slot programs; 1: every register step one fused 4-qubit unitary on
-        the matrix cores (complex64, specialised kernels)."""
+        """0: slot programs (a step fused only when its gates cost more than a
+        dense product on the vector units); 1: every register step one fused
+        4-qubit unitary on the matrix cores (complex64, specialised kernels);
+        2: a step fused when its gates cost more than that matrix-core product."""
         _check(lib.qsim_set_fusion(self._h, int(mode)))
 
     def set_jit(self, enable=True):
@@ -456,11 +458,12 @@ def device_memory():
 def jit_compile_check(num_qubits, arr, nops, arena, K, cmin, fuse=False, in_map=None):
     """Host-only: plan packed ops (after a pending layout change in_map, bit p
     -> in_map[p]) and compile every specialised pass kernel for gfx950 without
-    loading it; returns the number of passes."""
+    loading it; returns the number of passes.  fuse: a fusion mode
+    (Statevector.set_fusion) or a bool for mode 1."""
     code = C128 if arena.dtype == np.complex128 else C64
     m = None if in_map is None else np.ascontiguousarray(in_map, dtype=np.int32)
     npass = lib.qsim_jit_compile_check(num_qubits, code, ctypes.addressof(arr), nops, arena.ctypes.data,
-                                       arena.size, K, cmin, 1 if fuse else 0,
+                                       arena.size, K, cmin, int(fuse),
                                        None if m is None else m.ctypes.data)
     if npass < 0:
         raise RuntimeError(lib.qsim_last_error().decode())

@@ -141,7 +141,7 @@ struct qsim_state {
     bool perm_pending = false;  // perm_map is not the identity
     int32_t perm_map[64];
     bool jit = true;               // specialise forward chunk passes at run time
-    bool fuse = false;             // every register step one 16x16 unitary (matrix cores)
+    int fuse = 0;                  // fusion mode (qsim_set_fusion)
     std::string arch;              // device gcnArchName (for run-time compiles)
     // per-kernel-class timing with events on `stream` (off by default)
     bool prof = false, capturing = false;
@@ -371,7 +371,7 @@ int plan_for(qsim_state *st, const qsim_op *ops, int nops, const void *mats, int
     // amplitudes each
     const int K = max_probes > 1 ? std::min(st->K, st->dtype == QSIM_C64 ? 12 : 11) : st->K;
     if (!plan_passes(st->n, pops.data(), nops, mats, (int)amp_bytes(st->dtype), K, st->cmin, max_probes, plan,
-                     err, st->fuse && max_probes == 1, in_map))
+                     err, max_probes == 1 ? st->fuse : 0, in_map))
         return fail(err);
     return 0;
 }
@@ -533,7 +533,7 @@ int run_graphed(qsim_state *st, const qsim_op *ops, int nops, const void *mats, 
     put(&st->K, sizeof(int));
     put(&st->cmin, sizeof(int));
     put(&st->jit, sizeof(bool));
-    put(&st->fuse, sizeof(bool));
+    put(&st->fuse, sizeof(int));
     const bool keep = can_fold(st);
     put(&keep, sizeof(bool));
     if (keep) put(st->perm_map, sizeof(int32_t) * st->n);
@@ -1143,8 +1143,8 @@ int qsim_set_jit(qsim_state *st, int enable)
 int qsim_set_fusion(qsim_state *st, int mode)
 {
     if (!st) return fail("null state");
-    if (mode != 0 && mode != 1) return fail("fusion mode must be 0 or 1");
-    st->fuse = mode == 1;
+    if (mode < 0 || mode > 2) return fail("fusion mode must be 0, 1 or 2");
+    st->fuse = mode;
     return 0;
 }
 
@@ -1160,7 +1160,7 @@ int qsim_jit_compile_check(int num_qubits, int dtype, const qsim_op *ops, int no
     Plan plan;
     std::string err;
     if (!plan_passes(num_qubits, pops.data(), nops, mats, (int)amp_bytes(dtype), K, cmin,
-                     nv == 2 ? kMaxProbes : 1, plan, err, fuse != 0 && nv == 1,
+                     nv == 2 ? kMaxProbes : 1, plan, err, nv == 1 ? fuse : 0,
                      nv == 1 ? in_map : nullptr))
         return fail(err);
     if (!jit_compile_only(plan, dtype == QSIM_C128, nv, "gfx950", err))

@@ -422,7 +422,7 @@ void to_dense(Ctx &c, StepBuilder &sb)
 }  // namespace
 
 bool plan_passes(int n, const PlanOp *ops, int nops, const void *mats, int elem_bytes, int K, int cmin,
-                 int max_probes, Plan &out, std::string &err, bool fuse, const int32_t *in_map)
+                 int max_probes, Plan &out, std::string &err, int fuse, const int32_t *in_map)
 {
     out = Plan();
     if (n < kG || n > 63) {
@@ -525,7 +525,7 @@ bool plan_passes(int n, const PlanOp *ops, int nops, const void *mats, int elem_
         while (!srem.empty()) {
             StepBuilder sb;
             // fused plans: every step is one 16x16 product of all its ops
-            if (fuse && !any_probe) to_dense(ctx, sb);
+            if (fuse == 1 && !any_probe) to_dense(ctx, sb);
             uint64_t sblk = 0;
             std::vector<int> sleft;
             for (int idx : srem) {
@@ -584,7 +584,7 @@ bool plan_passes(int n, const PlanOp *ops, int nops, const void *mats, int elem_
                 }
             }
             // a slot step whose arithmetic exceeds one dense product becomes dense
-            if (!sb.dense && !sb.has_probe && sb.cost > 64.0) to_dense(ctx, sb);
+            if (!sb.dense && !sb.has_probe && sb.cost > (fuse == 2 ? 32.0 : 64.0)) to_dense(ctx, sb);
             // fill unused register bits with the lowest free chunk bits
             for (int b = 0; b < K && sb.nreg < kG; ++b)
                 if (sb.reg_of(b) < 0) sb.regbit[sb.nreg++] = b;

@@ -63,13 +63,17 @@ struct Plan {
 // as complex elements of `elem_bytes` (8: complex64, 16: complex128).  K is
 // clamped to n; cmin to K-2.  max_probes bounds the probe slots of one pass.
 // Returns false and sets err on invalid input.
-// fuse: every register step becomes one dense 16x16 matrix, the product of
-// all its ops (plans without probes only).
+// fuse (plans without probes only): 0, a slot step becomes one dense 16x16
+// matrix (the product of all its ops) when its slot arithmetic exceeds the
+// product's 64 multiply-adds per amplitude on the vector units; 1, every
+// register step becomes dense; 2, as 0 with the bar at 32 multiply-adds, the
+// cost of the product on the FP32 matrix cores (48 real multiply-adds per
+// amplitude at twice the vector rate).
 // in_map (n entries, or null): the input is in another layout (see
 // Plan::in_map); the first pass also holds the images of the lowest cmin
 // input bit positions, so it reads runs of 2^cmin contiguous amplitudes.
 bool plan_passes(int n, const PlanOp *ops, int nops, const void *mats, int elem_bytes, int K,
-                 int cmin, int max_probes, Plan &out, std::string &err, bool fuse = false,
+                 int cmin, int max_probes, Plan &out, std::string &err, int fuse = 0,
                  const int32_t *in_map = nullptr);
 
 // Pauli chunk passes: each term's X support must lie inside its pass's chunk.

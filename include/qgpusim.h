@@ -179,13 +179,18 @@ int qsim_jit_compile_check(int num_qubits, int dtype, const qsim_op *ops, int no
                            const void *mats, uint64_t mat_count, int K, int cmin, int fuse,
                            const int32_t *in_map);
 
-/* No reference counterpart.  Fusion mode of forward chunk passes:
- * 0 (default): slot programs (each gate by its own cheap arithmetic);
- * 1: every register step is fused into one 4-qubit (16x16) unitary, the
- * complex128 product of its gates rounded once, applied to the chunk as a
- * complex GEMM on the FP32 matrix cores (three real 16x16x4 MFMA products
- * per 16 columns, FP32 accumulation) when the pass kernel is specialised and
- * the state is complex64, else by the vector units. */
+/* No reference counterpart.  Fusion mode of forward chunk passes.  A fused
+ * register step applies one 4-qubit (16x16) unitary, the complex128 product
+ * of its gates rounded once, as a complex GEMM on the FP32 matrix cores
+ * (three real 16x16x4 MFMA products per 16 columns, FP32 accumulation) when
+ * the pass kernel is specialised and the state is complex64, else by the
+ * vector units.
+ * 0 (default): slot programs (each gate by its own cheap arithmetic); a step
+ *    is fused when its gates cost more than a dense product on the vector
+ *    units (64 multiply-adds per amplitude);
+ * 1: every register step is fused;
+ * 2: a step is fused when its gates cost more than the product on the matrix
+ *    cores (32 vector multiply-adds per amplitude). */
 int qsim_set_fusion(qsim_state *st, int mode);
 
 /* No reference counterpart (its custom backend has no sampling task):

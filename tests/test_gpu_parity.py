@@ -457,10 +457,13 @@ def test_batch_matches_single_circuit_runs():
 
 @pytest.mark.parametrize("dtype,K", [(np.complex64, 12), (np.complex64, 9), (np.complex64, 13), (np.complex128, 11)])
 @pytest.mark.parametrize("circ", ["random", "qft", "mixed"])
-def test_fused_steps_match_oracle(hip, dtype, K, circ):
+@pytest.mark.parametrize("mode", [1, 2])
+def test_fused_steps_match_oracle(hip, dtype, K, circ, mode):
     """Fusion mode 1: every register step is one 16x16 unitary (the product of
-    its gates); complex64 specialised kernels apply it on the matrix cores
-    (3M FP32 MFMA), the slot kernel and complex128 by the vector units."""
+    its gates); mode 2: only the steps whose gates cost more than that product
+    on the matrix cores.  complex64 specialised kernels apply fused steps on
+    the matrix cores (3M FP32 MFMA), the slot kernel and complex128 by the
+    vector units."""
     n = 14
     if circ == "random":
         ops = _tuples(C.random_circuit(n, 12, 77 + K))
@@ -476,7 +479,7 @@ def test_fused_steps_match_oracle(hip, dtype, K, circ):
     for jit in (True, False):
         sv = _sv(hip, n, dtype)
         sv.set_tile_params(K, 4 if dtype == np.complex64 else 3)
-        sv.set_fusion(1)
+        sv.set_fusion(mode)
         sv.set_jit(jit)
         sv.from_numpy(psi0)
         sv.apply_tile_cooperative(ops)
