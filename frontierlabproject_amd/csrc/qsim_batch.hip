@@ -141,7 +141,12 @@ __global__ __launch_bounds__(kTPB) void k_batch_reset(unsigned *counter)
 
 // Every workgroup loops: take the next queue position, run that circuit,
 // record its result and wall-clock interval; exit once the queue is empty.
-__global__ __launch_bounds__(kTPB) void k_batch(BatchArgs A)
+// Two waves per SIMD: the LDS (two chunks, ~68 KB) allows two workgroups per
+// CU, and the register cap that allows it (256, with ~110 registers spilled
+// in the two-vector steps) still lifts the 4096-circuit batch from 660 to
+// 1047 circuits/s (measured) by overlapping one workgroup's chunk copies
+// with the other's steps.
+__global__ __launch_bounds__(kTPB) __attribute__((amdgpu_waves_per_eu(2))) void k_batch(BatchArgs A)
 {
     __shared__ __align__(16) cx<float> c0[1 << kBatchK];
     __shared__ __align__(16) cx<float> c1[1 << kBatchK];
