@@ -153,9 +153,11 @@ def run_custom_backend(qc, tile_plan: List[Tile], task: str, shots: int = None,
     num_tiles, num_gates, num_fused_ops, fusion_ratio, num_reorders,
     num_kernel_launches, num_graph_replays, avoided_launches, t_perm_s,
     t_apply_s, t_sync_s, t_total_s, t_launch_overhead_saved_us, tile_mode.
-    ``permute`` is lazy (folded into the next tile's first chunk pass), so
-    t_perm_s holds only its host side and the permutation work is in
-    t_apply_s.  With ``task="sampling"`` and ``shots``, ``metrics["counts"]``
+    ``permute`` moves no data (it composes the state's stored bit order; the
+    next tile's first chunk pass moves only the qubits that tile needs low,
+    and the last layout is restored when the state is read), so t_perm_s
+    holds only its host side and the data movement is in t_apply_s and
+    t_sync_s.  With ``task="sampling"`` and ``shots``, ``metrics["counts"]``
     holds outcomes sampled on the GPU from the final state (seed
     ``sample_seed``) over the circuit's measurements, or all qubits.
     """
