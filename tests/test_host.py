@@ -255,6 +255,59 @@ def test_chunk_plan_order_is_equivalent(hip, seed):
     assert np.max(abs(ref - got)) < 1e-12
 
 
+def _in_order_passes(masks, K, low):
+    """The in-order rule: each pass takes ops in program order while the chunk
+    (the `low` bits plus the ops' qubits) has at most K bits; an op that does
+    not fit blocks its qubits for the rest of the pass."""
+    live = [True] * len(masks)
+    passes = 0
+    while any(live):
+        P, blocked = low, 0
+        for i, m in enumerate(masks):
+            if not live[i]:
+                continue
+            if (m & blocked) or bin(P | m).count("1") > K:
+                blocked |= m
+                continue
+            P |= m
+            live[i] = False
+        passes += 1
+    return passes
+
+
+def test_chunk_search_never_worse_than_in_order(hip):
+    """The chunk-bit search never needs more passes than the in-order rule on
+    tile-sized op lists (18 qubits, 13-bit chunks, 4 row bits), and saves
+    passes on some; executing its order still gives the program-order state
+    (checked on 10-qubit lists, where the dense oracle is cheap)."""
+    rng = np.random.default_rng(5)
+    fewer = 0
+    for trial in range(24):
+        n, K, cmin = (18, 13, 4) if trial % 3 else (10, 7, 2)
+        ops = C.random_circuit(n, int(rng.integers(2, 5)), 500 + trial)
+        tup = _tuples(ops)
+        arr, arena = hip.pack_ops(tup, n, np.complex128)
+        passes, order = hip.plan_describe(n, arr, len(tup), arena, K, cmin)
+        masks = [sum(1 << q for q in o.qubits) for o in ops]
+        greedy = _in_order_passes(masks, K, (1 << cmin) - 1)
+        assert len(passes) <= greedy
+        fewer += len(passes) < greedy
+        assert sorted(order) == list(range(len(tup)))
+        if n <= 10:
+            g = C.to_dense_gates(ops)
+            assert np.max(abs(dense.simulate(g, n) - dense.simulate([g[i] for i in order], n))) < 1e-12
+    assert fewer > 0
+
+
+def test_random33_tiles_plan_in_two_passes(hip):
+    """Every tile of the 33-qubit depth-30 program (18 qubits each, so two
+    13-bit chunks at least) plans in exactly two passes."""
+    import bench
+    qc, plan, prog, plain, ng, tt = bench.build_program(hip, 33, 30, 18, 1002)
+    counts = [len(hip.plan_describe(33, it[1], it[2], it[3], 13, 4)[0]) for it in prog if it[0] == "tile"]
+    assert len(counts) == 19 and all(c == 2 for c in counts)
+
+
 def test_chunk_plan_keeps_probe_order(hip):
     """A probe never moves across an op on its own qubit."""
     n = 8
