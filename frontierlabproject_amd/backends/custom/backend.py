@@ -7,8 +7,10 @@ Differences from the reference, each deliberate:
 
 * the gate-matrix cache is keyed on (name, params, dtype); the reference's key
   omits the dtype (backend.py:30-35);
-* ops on three or more qubits raise ``ValueError`` instead of being dropped
-  silently (backend.py:448-451);
+* three-qubit gates (ccx, ccz, cswap) run as their exact 1q/2q networks
+  inside their tile (all three qubits are local there); other ops on three or
+  more qubits raise ``ValueError``; the reference drops both silently
+  (backend.py:448-451);
 * fused matrices are multiplied in complex128 and then cast;
 * there is no CuPy fallback simulator: the HIP library is required.
 
@@ -24,6 +26,7 @@ from typing import List
 
 import numpy as np
 
+from ...circuit import DECOMPOSITIONS, decompose
 from ...transpiler.passes import Tile
 from .native import qgpusim_hip as qgpusim_cuda
 
@@ -138,9 +141,13 @@ def build_tile_ops(tile: Tile, qubit_map, np_dtype):
             ops.append(("1q", local[0], U))
         elif len(local) == 2:
             ops.append(("2q", local[0], local[1], U))
+        elif op.name in DECOMPOSITIONS:
+            for g, qs in decompose(op.name, local):
+                Ug = get_gate_matrix_host(g, np_dtype)
+                ops.append(("1q", qs[0], Ug) if len(qs) == 1 else ("2q", qs[0], qs[1], Ug))
         else:
             raise ValueError(f"gate {op.name!r} acts on {len(local)} qubits; "
-                             "the tile path supports 1- and 2-qubit gates")
+                             "the tile path supports 1- and 2-qubit gates and ccx, ccz, cswap")
     return ops, ngates
 
 

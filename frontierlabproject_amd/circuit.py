@@ -165,9 +165,46 @@ _MATRIX = {
                              dtype=np.complex128),
 }
 
-_NUM_QUBITS = {name: (2 if name in ("cx", "cy", "cz", "cp", "crz", "swap") else 1) for name in _MATRIX}
+def _perm3(a, b):
+    """3-qubit permutation matrix exchanging basis states a and b."""
+    m = np.eye(8, dtype=np.complex128)
+    m[[a, b]] = m[[b, a]]
+    return m
 
-_SELF_INVERSE = ("id", "h", "x", "y", "z", "cx", "cy", "cz", "swap")
+
+# Three-qubit gates, index b_q0 + 2 b_q1 + 4 b_q2 (little-endian like the rest):
+# ccx flips qargs[2] when qargs[0] and qargs[1] are 1 (|011> <-> |111>), ccz
+# negates |111>, cswap exchanges qargs[1] and qargs[2] when qargs[0] is 1
+# (|011> <-> |101>).
+_MATRIX.update({
+    "ccx": lambda: _perm3(3, 7),
+    "ccz": lambda: np.diag([1, 1, 1, 1, 1, 1, 1, -1]).astype(np.complex128),
+    "cswap": lambda: _perm3(3, 5),
+})
+
+_NUM_QUBITS = {name: (2 if name in ("cx", "cy", "cz", "cp", "crz", "swap") else 3 if name in ("ccx", "ccz", "cswap")
+                      else 1) for name in _MATRIX}
+
+# Exact 1q/2q networks of the three-qubit gates (no global phase): the
+# Toffoli network of six CNOTs, T/T^dag and two Hadamards (Nielsen & Chuang,
+# Fig. 4.9); CCZ = H_t CCX H_t; CSWAP = CX(b, a) CCX(c, a, b) CX(b, a).
+# Entries are (gate name, positions into the gate's qargs).
+_CCX_NET = [("h", (2,)), ("cx", (1, 2)), ("tdg", (2,)), ("cx", (0, 2)), ("t", (2,)), ("cx", (1, 2)),
+            ("tdg", (2,)), ("cx", (0, 2)), ("t", (1,)), ("t", (2,)), ("h", (2,)), ("cx", (0, 1)),
+            ("t", (0,)), ("tdg", (1,)), ("cx", (0, 1))]
+DECOMPOSITIONS = {
+    "ccx": _CCX_NET,
+    "ccz": [("h", (2,))] + _CCX_NET + [("h", (2,))],
+    "cswap": [("cx", (2, 1))] + _CCX_NET + [("cx", (2, 1))],
+}
+
+
+def decompose(name, qargs):
+    """[(gate, qubits)] of the 1q/2q network for a three-qubit gate on qargs."""
+    return [(gate(g), [qargs[i] for i in q]) for g, q in DECOMPOSITIONS[name]]
+
+
+_SELF_INVERSE = ("id", "h", "x", "y", "z", "cx", "cy", "cz", "swap", "ccx", "ccz", "cswap")
 _INVERSE = {name: (lambda p, n=name: (n, p)) for name in _SELF_INVERSE}
 _INVERSE.update({
     "s": lambda p: ("sdg", []), "sdg": lambda p: ("s", []),
@@ -249,6 +286,9 @@ class QuantumCircuit:
     def cp(self, lam, c, t): return self._g("cp", [c, t], lam)
     def crz(self, theta, c, t): return self._g("crz", [c, t], theta)
     def swap(self, a, b): return self._g("swap", [a, b])
+    def ccx(self, c0, c1, t): return self._g("ccx", [c0, c1, t])
+    def ccz(self, a, b, c): return self._g("ccz", [a, b, c])
+    def cswap(self, c, a, b): return self._g("cswap", [c, a, b])
 
     def unitary(self, matrix, qargs):
         return self.append(UnitaryGate(matrix), qargs)
