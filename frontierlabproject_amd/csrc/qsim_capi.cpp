@@ -140,6 +140,9 @@ struct qsim_state {
     // the low bits inside their first chunk pass; readers apply it first.
     bool perm_pending = false;  // perm_map is not the identity
     int32_t perm_map[64];
+    // The state is |0...0> but not written yet (reset): the next chunked op
+    // list's first pass generates it; any other use writes it first.
+    bool zero_pending = false;
     bool jit = true;               // specialise forward chunk passes at run time
     int fuse = 0;                  // fusion mode (qsim_set_fusion)
     std::string arch;              // device gcnArchName (for run-time compiles)
@@ -200,9 +203,24 @@ int ensure_tmp(qsim_state *st)
 
 int build_perm_args(int n, const int32_t *map, int smax, PermArgs &P);
 
-// Applies a pending layout change with the permutation kernel.
+// Writes a pending |0...0>.
+int materialize_zero(qsim_state *st)
+{
+    if (!st->zero_pending) return 0;
+    if (st->dtype == QSIM_C64)
+        HIP_TRY(launch_init_basis<float>(static_cast<cx<float> *>(st->psi), st->dim, 0, st->stream));
+    else
+        HIP_TRY(launch_init_basis<double>(static_cast<cx<double> *>(st->psi), st->dim, 0, st->stream));
+    st->zero_pending = false;
+    st->counters[0]++;
+    return 0;
+}
+
+// Brings the buffer to the logical state: writes a pending |0...0>, applies a
+// stored order that differs from the logical one with the permutation kernel.
 int flush_perm(qsim_state *st)
 {
+    if (materialize_zero(st)) return -1;
     if (!st->perm_pending) return 0;
     if (ensure_tmp(st)) return -1;
     PermArgs P;
@@ -467,7 +485,12 @@ int plan_and_compile(qsim_state *st, const qsim_op *ops, int nops, const void *m
     after.pending = false;
     if (!keep) {
         if (plan_for(st, ops, nops, mats, 1, plan)) return -1;
-        return jit_for(st, plan, sizeof(R) == 8, 1, fns);
+        // a pending |0...0> is generated by a specialised first pass
+        plan.zero_input = st->zero_pending && st->jit;
+        if (jit_for(st, plan, sizeof(R) == 8, 1, fns)) return -1;
+        if (plan.zero_input && (fns.empty() || !fns[0])) plan.zero_input = false;
+        if (st->zero_pending && !plan.zero_input && materialize_zero(st)) return -1;
+        return 0;
     }
     const int n = st->n;
     int32_t tgt[64], in_map[64];
@@ -514,6 +537,7 @@ int run_chunked(qsim_state *st, const qsim_op *ops, int nops, const void *mats, 
     if (st->up.upload(img.bytes.data(), img.bytes.size(), st->stream, &dev)) return -1;
     if (enqueue_passes<R>(st, plan, img, static_cast<char *>(dev), nullptr, nullptr, &fns)) return -1;
     set_layout(st, after);
+    st->zero_pending = false;
     return 0;
 }
 
@@ -521,8 +545,10 @@ template <typename R>
 int run_graphed(qsim_state *st, const qsim_op *ops, int nops, const void *mats, uint64_t mat_count)
 {
     // a layout change that cannot fold is applied first, so the key sees
-    // the buffer the passes will use
+    // the buffer the passes will use; so is a pending |0...0> that no
+    // specialised first pass will generate (the graph must not need it)
     if (st->perm_pending && !can_fold(st) && flush_perm(st)) return -1;
+    if (st->zero_pending && !st->jit && materialize_zero(st)) return -1;
     // key: buffer address, geometry, ops, matrices
     std::vector<char> key;
     auto put = [&key](const void *p, size_t b) {
@@ -537,6 +563,7 @@ int run_graphed(qsim_state *st, const qsim_op *ops, int nops, const void *mats, 
     const bool keep = can_fold(st);
     put(&keep, sizeof(bool));
     if (keep) put(st->perm_map, sizeof(int32_t) * st->n);
+    put(&st->zero_pending, sizeof(bool));
     put(ops, sizeof(qsim_op) * nops);
     put(mats, sizeof(cx<R>) * mat_count);
     for (auto it = st->graphs.begin(); it != st->graphs.end(); ++it) {
@@ -550,6 +577,7 @@ int run_graphed(qsim_state *st, const qsim_op *ops, int nops, const void *mats, 
             }
             st->perm_pending = g.after_pending;
             std::memcpy(st->perm_map, g.after_map, sizeof(int32_t) * st->n);
+            st->zero_pending = false;
             st->counters[1]++;
             st->counters[3] += 0;
             return 0;
@@ -593,6 +621,7 @@ int run_graphed(qsim_state *st, const qsim_op *ops, int nops, const void *mats, 
     HIP_TRY(hipGraphLaunch(st->graphs.front().exec, st->stream));
     st->counters[1]++;
     set_layout(st, after);
+    st->zero_pending = false;
     while (st->graphs.size() > kMaxGraphs) {
         HIP_TRY(hipStreamSynchronize(st->stream));
         GraphEntry &old = st->graphs.back();
@@ -874,11 +903,7 @@ int qsim_reset(qsim_state *st)
 {
     if (!st) return fail("null state");
     st->perm_pending = false;  // the state is overwritten
-    if (st->dtype == QSIM_C64)
-        HIP_TRY(launch_init_basis<float>(static_cast<cx<float> *>(st->psi), st->dim, 0, st->stream));
-    else
-        HIP_TRY(launch_init_basis<double>(static_cast<cx<double> *>(st->psi), st->dim, 0, st->stream));
-    st->counters[0]++;
+    st->zero_pending = true;   // written by the next first pass, or by the next reader
     return 0;
 }
 
@@ -887,6 +912,7 @@ int qsim_from_host(qsim_state *st, const void *src, uint64_t count)
     if (!st) return fail("null state");
     if (count != st->dim) return fail("Input array must have size 2^num_qubits");
     st->perm_pending = false;  // the state is overwritten
+    st->zero_pending = false;
     HIP_TRY(hipMemcpyAsync(st->psi, src, count * amp_bytes(st->dtype), hipMemcpyHostToDevice, st->stream));
     HIP_TRY(hipStreamSynchronize(st->stream));
     return 0;
@@ -944,6 +970,7 @@ int qsim_permute(qsim_state *st, const int32_t *layout_old, const int32_t *layou
     if (num_qubits != st->n) return fail("layout_old/layout_new must have length num_qubits");
     int32_t map[64];
     if (permute_map_impl(st->n, layout_old, layout_new, map)) return -1;
+    if (st->zero_pending) return 0;  // a bit permutation leaves |0...0> as it is
     // Recorded, not run: the next op list folds it into its first chunk pass
     // (or it runs as one permutation pass when anything else reads the state).
     if (st->perm_pending) {

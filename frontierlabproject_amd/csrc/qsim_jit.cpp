@@ -350,7 +350,14 @@ std::string jit_pass_source(const Plan &plan, size_t p, bool dbl, int nv, const 
     o << "  const uint64_t base = 0ull" << dep.str() << ";\n";
     o << "  __syncthreads();\n";
     const std::string ntf = ", true, " + std::to_string(TPB);  // non-temporal (streamed once), block size
-    o << "  chunk_copy<" << T << ", true" << ntf << ">(psi, c0, base, rowoff, " << E << ", " << CF << ");\n";
+    if (p == 0 && plan.zero_input && nv == 1) {
+        // the input is |0...0>: the chunk holding amplitude 0 starts as e_0, every other as zeros
+        o << "  for (int t = threadIdx.x; t < " << E << "; t += " << TPB << ") c0[t] = cx<" << T << ">{0, 0};\n";
+        o << "  __syncthreads();\n";
+        o << "  if (base == 0 && threadIdx.x == 0) c0[swz(0)] = cx<" << T << ">{1, 0};\n";
+    } else {
+        o << "  chunk_copy<" << T << ", true" << ntf << ">(psi, c0, base, rowoff, " << E << ", " << CF << ");\n";
+    }
     if (nv == 2) o << "  chunk_copy<" << T << ", true" << ntf << ">(lam, c1, base, rowoff, " << E << ", " << CF << ");\n";
     o << "  __syncthreads();\n";
     for (int s = 0; s < pd.nsteps; ++s) emit_step(o, plan.steps[pd.step_off + s], E, T, nv, TPB);

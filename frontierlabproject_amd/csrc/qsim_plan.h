@@ -52,6 +52,9 @@ struct Plan {
     // then reads the input in the old layout and writes its output, in the
     // pass layout, to another buffer.
     std::vector<int32_t> in_map;
+    // When set (the input is |0...0>, not yet written): the first pass
+    // generates its chunks instead of reading them.
+    bool zero_input = false;
 };
 
 // Scalar factors (a diagonal gate's first entry, H's 1/sqrt2) are taken out of

@@ -69,7 +69,10 @@ int qsim_dtype(const qsim_state *st);
 /* Device address of the amplitudes (changes after qsim_permute). */
 int qsim_device_ptr(qsim_state *st, void **out);
 
-/* reset() -> |0...0> (bindings.cpp:577-582). */
+/* reset() -> |0...0> (bindings.cpp:577-582).  Written lazily: the next op
+ * list in a chunked mode with specialised kernels generates |0...0> in its
+ * first pass instead of reading it; any other use of the state writes it
+ * first.  qsim_create starts in this state. */
 int qsim_reset(qsim_state *st);
 
 /* from_numpy / to_numpy (bindings.cpp:547-574).  `count` is the number of

@@ -623,6 +623,37 @@ def test_stored_layout_moves_op_qubits_low(hip, mode, dtype):
 
 
 @pytest.mark.parametrize("dtype", [np.complex64, np.complex128])
+@pytest.mark.parametrize("jit", [True, False])
+@pytest.mark.parametrize("mode", ["apply_tile_cooperative", "apply_tile_graphed", "apply_tile_dev"])
+def test_lazy_reset(hip, dtype, jit, mode):
+    """reset() leaves |0...0> unwritten: a specialised first pass generates it,
+    every other path writes it first.  Fresh states, states reset after other
+    work, permuted zero states and graph replays all match the oracle."""
+    n = 13
+    ops = _tuples(C.random_circuit(n, 4, 61))
+    e0 = np.zeros(1 << n, dtype=np.complex128)
+    e0[0] = 1
+    ref = e0
+    for o in ops:
+        ref = dense.apply(ref, o[-1], list(o[1:-1]), n)
+    rng = np.random.default_rng(3)
+    L = _layouts(n, rng, 1)
+    sv = _sv(hip, n, dtype)
+    sv.set_jit(jit)
+    assert abs(sv.amplitude(0) - 1) < 1e-7 and abs(sv.amplitude(5)) == 0  # fresh state reads as |0>
+    for rep in range(3):
+        sv.from_numpy(C.random_state(n, 20 + rep))  # garbage that reset must discard
+        sv.reset()
+        if rep == 1:
+            sv.permute(L[0], L[1])  # |0...0> is invariant under a bit permutation
+            sv.permute(L[1], L[0])
+        getattr(sv, mode)(ops)
+        _check(ref, sv.to_numpy(), dtype)
+    sv.reset()
+    _check(e0, sv.to_numpy(), dtype)
+
+
+@pytest.mark.parametrize("dtype", [np.complex64, np.complex128])
 def test_pending_permutation_flushes(hip, dtype):
     """Every reader of the state sees the permuted state."""
     n = 12
