@@ -204,7 +204,7 @@ def main():
         return main_batch(args, hip, np)
     c128 = args.dtype == "c128"
     cdt = np.complex128 if c128 else np.complex64
-    if c128:  # config 2 in complex128: the tile path, its kernels and the mirror echo
+    if c128:  # the same circuit in complex128: the tile path, its kernels and the mirror echo
         args.no_plain = args.no_adjoint = args.no_cpu = True
     n = args.n
     bytes_per_pass = 2 * (1 << n) * (16 if c128 else 8)
@@ -531,7 +531,7 @@ def main_batch(args, hip, np):
         if len(g_ref):
             max_dg = max(max_dg, float(np.max(np.abs(g[i] - g_ref))))
     ns = np.array([s[0] for s in spec])
-    out = {"workload": "4096 random circuits (12-22 qubits, depth 10-40, config-0 gate family, per-circuit seeds), "
+    out = {"workload": "4096 random circuits (12-22 qubits, depth 10-40, the random-circuit gate family, per-circuit seeds), "
                        "c64, <Z_0> + adjoint gradient each, one persistent launch, hipGraph replay",
            "circuits_per_s": round(ncirc / t_launch, 1), "batch_ms": round(t_launch * 1e3, 3),
            "batch_wall_ms": round(float(np.median(walls)) * 1e3, 3),

@@ -150,7 +150,7 @@ def test_special_pair_gates(hip):
 @pytest.mark.parametrize("use_float32", [True, False])
 @pytest.mark.parametrize("tile_mode", ["sequential", "cooperative", "graphed"])
 def test_run_custom_backend_random(use_float32, tile_mode):
-    """Config-0 shape (random circuit, marker, LQR) at 14 qubits."""
+    """The random circuit (save marker, LQR) at 14 qubits."""
     n, depth, nL = 14, 12, 7
     qc = W.random_circuit(n, depth, 1000, measure_marker=True)
     pm, p = make_custom_pm(nL)
@@ -201,8 +201,9 @@ def test_three_qubit_gates_run(use_float32, tile_mode):
     _check(ref, psi, np.complex64 if use_float32 else np.complex128)
 
 
-def test_config0_random_20q_c128_nl10():
-    """BASELINE config 0 at full size: 20 qubits, depth 20, complex128, nL=10."""
+def test_random_20q_c128_nl10():
+    """The 20-qubit depth-20 random circuit in complex128 with nL=10, in all
+    three tile modes, against the FP64 oracle."""
     n = 20
     qc = W.random_circuit(n, 20, 1000, measure_marker=True)
     pm, p = make_custom_pm(10)
@@ -212,7 +213,7 @@ def test_config0_random_20q_c128_nl10():
         psi, m = run_custom_backend(tqc, p.property_set["flat_tiling_plan"], "statevector",
                                     use_float32=False, tile_mode=mode)
         f = dense.fidelity(ref, psi)
-        print(f"config0 mode={mode} 1-F={1 - f:.3e} maxabs={dense.max_abs_err_phase_aligned(ref, psi):.3e}")
+        print(f"random 20q mode={mode} 1-F={1 - f:.3e} maxabs={dense.max_abs_err_phase_aligned(ref, psi):.3e}")
         assert 1 - f <= 1e-10
 
 
