@@ -78,6 +78,10 @@ enum : int32_t {
     S1_Z = 11,         // y' = -y
     S1_PI = 12,        // y' = i y   (S)
     S1_MI = 13,        // y' = -i y  (S^dag)
+    // a real factor u0 taken out (plans without probes, |u0| >= the off-diagonal):
+    S1_REAL1 = 14,     // [[1, a],[b, 1]], a b real               1 cx (a,b)
+    S1_RI1 = 15,       // [[1, iq],[ir, 1]], q r real             1 cx (q,r)
+    S1_GEN1 = 16,      // [[1, b],[c, d]]  (u0 complex, taken out) 3 cx
 };
 // pair slots (matrix index b_first + 2 b_second, first < second register bit);
 // other pair ops make the step dense:

@@ -153,6 +153,42 @@ __device__ __forceinline__ void apply1(cx<R> *a, int kind, const cx<R> *__restri
             }
         break;
     }
+    case S1_REAL1: {  // x' = x + a y, y' = b x + y
+        const cx<R> ab = mp[0];
+#pragma unroll
+        for (int m = 0; m < 16; ++m)
+            if (!((m >> B) & 1)) {
+                const int m1 = m | (1 << B);
+                const cx<R> x = a[m], y = a[m1];
+                a[m] = cx<R>{fma(ab.re, y.re, x.re), fma(ab.re, y.im, x.im)};
+                a[m1] = cx<R>{fma(ab.im, x.re, y.re), fma(ab.im, x.im, y.im)};
+            }
+        break;
+    }
+    case S1_RI1: {  // x' = x + i q y, y' = i r x + y
+        const cx<R> qr = mp[0];
+#pragma unroll
+        for (int m = 0; m < 16; ++m)
+            if (!((m >> B) & 1)) {
+                const int m1 = m | (1 << B);
+                const cx<R> x = a[m], y = a[m1];
+                a[m] = cx<R>{fma(-qr.re, y.im, x.re), fma(qr.re, y.re, x.im)};
+                a[m1] = cx<R>{fma(-qr.im, x.im, y.re), fma(qr.im, x.re, y.im)};
+            }
+        break;
+    }
+    case S1_GEN1: {  // x' = x + b y, y' = c x + d y
+        const cx<R> b = mp[0], c = mp[1], d = mp[2];
+#pragma unroll
+        for (int m = 0; m < 16; ++m)
+            if (!((m >> B) & 1)) {
+                const int m1 = m | (1 << B);
+                const cx<R> x = a[m], y = a[m1];
+                a[m] = cmad(b, y, x);
+                a[m1] = cmad(c, x, cmul(d, y));
+            }
+        break;
+    }
     case S1_X: {
 #pragma unroll
         for (int m = 0; m < 16; ++m)

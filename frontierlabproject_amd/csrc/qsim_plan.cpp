@@ -233,6 +233,14 @@ int push_mats(Plan &p, const Cd *v, int count)
 
 // Place op `idx` (register bits rx, ry already assigned) into the step.
 // Returns 1 when placed, 0 when no slot is free, 2 when the op needs a dense step.
+// u0 can be taken out of a 1q matrix without growing its other entries:
+// |u0| is at least |u1| and |u2| (at least 1/sqrt2 for a unitary).
+inline bool factorable(const Cd *u)
+{
+    const double a = u[0].re * u[0].re + u[0].im * u[0].im;
+    return a > 0.0 && a >= u[1].re * u[1].re + u[1].im * u[1].im && a >= u[2].re * u[2].re + u[2].im * u[2].im;
+}
+
 int place(Ctx &c, StepBuilder &sb, int idx, int rx, int ry)
 {
     const PlanOp &op = c.ops[idx];
@@ -302,15 +310,40 @@ int place(Ctx &c, StepBuilder &sb, int idx, int rx, int ry)
             sb.sd.kind[j] = S1_BFLY;
             sb.cost += 2.0;
         } else if (isre(u[0]) && isre(u[1]) && isre(u[2]) && isre(u[3])) {
-            Cd r[2] = {{u[0].re, u[1].re}, {u[2].re, u[3].re}};
-            sb.sd.kind[j] = S1_REAL;
-            sb.cost += 4.0;
-            sb.sd.mat[j] = push_mats(p, r, 2);
+            if (c.real_scale && u[0].re == u[3].re && factorable(u)) {
+                // u0 * [[1, a], [b, 1]] (RY): u0 applied later
+                Cd ab = {u[1].re / u[0].re, u[2].re / u[0].re};
+                sb.factor = cmulh(sb.factor, u[0]);
+                sb.sd.kind[j] = S1_REAL1;
+                sb.cost += 2.0;
+                sb.sd.mat[j] = push_mats(p, &ab, 1);
+            } else {
+                Cd r[2] = {{u[0].re, u[1].re}, {u[2].re, u[3].re}};
+                sb.sd.kind[j] = S1_REAL;
+                sb.cost += 4.0;
+                sb.sd.mat[j] = push_mats(p, r, 2);
+            }
         } else if (isre(u[0]) && isre(u[3]) && isim(u[1]) && isim(u[2])) {
-            Cd r[2] = {{u[0].re, u[1].im}, {u[2].im, u[3].re}};
-            sb.sd.kind[j] = S1_RI;
-            sb.cost += 4.0;
-            sb.sd.mat[j] = push_mats(p, r, 2);
+            if (c.real_scale && u[0].re == u[3].re && factorable(u)) {
+                // u0 * [[1, iq], [ir, 1]] (RX): u0 applied later
+                Cd qr = {u[1].im / u[0].re, u[2].im / u[0].re};
+                sb.factor = cmulh(sb.factor, u[0]);
+                sb.sd.kind[j] = S1_RI1;
+                sb.cost += 2.0;
+                sb.sd.mat[j] = push_mats(p, &qr, 1);
+            } else {
+                Cd r[2] = {{u[0].re, u[1].im}, {u[2].im, u[3].re}};
+                sb.sd.kind[j] = S1_RI;
+                sb.cost += 4.0;
+                sb.sd.mat[j] = push_mats(p, r, 2);
+            }
+        } else if (c.real_scale && factorable(u)) {
+            // u0 * [[1, b], [c, d]]: u0 applied later
+            Cd g[3] = {cdivh(u[1], u[0]), cdivh(u[2], u[0]), cdivh(u[3], u[0])};
+            sb.factor = cmulh(sb.factor, u[0]);
+            sb.sd.kind[j] = S1_GEN1;
+            sb.cost += 6.0;
+            sb.sd.mat[j] = push_mats(p, g, 3);
         } else {
             sb.sd.kind[j] = S1_GEN;
             sb.cost += 8.0;
