@@ -84,6 +84,7 @@ _SIGNATURES = {
     "qsim_batch_run": ([_P, _I, _P, _P, _P, _P], _I),
     "qsim_batch_info": ([_P, _P, _I], _I),
     "qsim_jit_compile_check": ([_I, _I, _P, _I, _P, _U64, _I, _I, _I, _P], _I),
+    "qsim_choose_layout": ([_I, _I, _I, _P, _P, _I, _P], _I),
     "qsim_set_fusion": ([_P, _I], _I),
     "qsim_device_memory": ([_P, _P], _I),
     "qsim_sample": ([_P, _U64, _U64, _P], _I),
@@ -468,6 +469,18 @@ def jit_compile_check(num_qubits, arr, nops, arena, K, cmin, fuse=False, in_map=
     if npass < 0:
         raise RuntimeError(lib.qsim_last_error().decode())
     return npass
+
+
+def choose_layout(num_qubits, perm_map, arr, nops, dtype=np.complex64, cmin=None):
+    """Host-only: stored position of each logical bit for an op list run on
+    the stored order perm_map (perm_map[p] = logical bit at stored bit p)."""
+    code = C128 if dtype == np.complex128 else C64
+    cm = (4 if code == C64 else 3) if cmin is None else cmin
+    pm = np.ascontiguousarray(perm_map, dtype=np.int32)
+    out = np.zeros(num_qubits, dtype=np.int32)
+    _check(lib.qsim_choose_layout(num_qubits, code, cm, pm.ctypes.data, ctypes.addressof(arr), nops,
+                                  out.ctypes.data))
+    return out.tolist()
 
 
 def plan_describe(num_qubits, arr, nops, arena, K, cmin, steps=False):

@@ -431,17 +431,17 @@ int low_region(const qsim_state *st) { return st->dtype == QSIM_C64 ? 18 : 17; }
 // position): the first chunk pass then scatters to near rows only, while
 // qubits outside its chunk move with the chunk base at no cost.  Every other
 // qubit stays.
-void choose_layout(const qsim_state *st, const qsim_op *ops, int nops, int32_t *tgt)
+void choose_layout_impl(int n, int cmin, int region, const int32_t *at, const qsim_op *ops, int nops,
+                        int32_t *tgt)
 {
-    const int n = st->n;
-    const int32_t *at = st->perm_map;  // logical bit held by stored bit p
+    // at[p]: logical bit held by stored bit p
     for (int p = 0; p < n; ++p) tgt[at[p]] = p;
     uint64_t T = 0;
     for (int i = 0; i < nops; ++i) {
         T |= 1ull << ops[i].q0;
         if (ops[i].kind == QSIM_OP_2Q) T |= 1ull << ops[i].q1;
     }
-    const int r = std::min(n, std::max(__builtin_popcountll(T), low_region(st)));
+    const int r = std::min(n, std::max(__builtin_popcountll(T), region));
     std::vector<int> slots, in;  // ascending stored positions
     for (int p = 0; p < r; ++p)
         if (!((T >> at[p]) & 1)) slots.push_back(p);
@@ -455,7 +455,7 @@ void choose_layout(const qsim_state *st, const qsim_op *ops, int nops, int32_t *
     for (size_t i = 0; i < in.size(); ++i) {
         const int slot = slots[i], leaving = at[slot];
         tgt[at[in[i]]] = slot;
-        if (slot < st->cmin) {
+        if (slot < cmin) {
             while (taken[scan]) ++scan;  // stays below n: at most in.size() positions are taken
             taken[scan] = true;
             tgt[leaving] = scan;
@@ -467,6 +467,11 @@ void choose_layout(const qsim_state *st, const qsim_op *ops, int nops, int32_t *
     size_t k = 0;
     for (int p : in)
         if (!taken[p]) tgt[movers[k++]] = p;
+}
+
+void choose_layout(const qsim_state *st, const qsim_op *ops, int nops, int32_t *tgt)
+{
+    choose_layout_impl(st->n, st->cmin, low_region(st), st->perm_map, ops, nops, tgt);
 }
 
 // Stored order after an op list: `after` (perm_map form) and whether it
@@ -1193,6 +1198,30 @@ int qsim_jit_compile_check(int num_qubits, int dtype, const qsim_op *ops, int no
     if (!jit_compile_only(plan, dtype == QSIM_C128, nv, "gfx950", err))
         return fail("run-time kernel compile: " + err);
     return (int)plan.passes.size();
+}
+
+int qsim_choose_layout(int num_qubits, int dtype, int cmin, const int32_t *perm_map, const qsim_op *ops, int nops,
+                       int32_t *tgt_out)
+{
+    if (num_qubits < 1 || num_qubits > 63) return fail("num_qubits must be between 1 and 63");
+    if (dtype != QSIM_C64 && dtype != QSIM_C128) return fail("unknown dtype");
+    if (cmin < 0 || cmin > num_qubits) return fail("cmin out of range");
+    if (nops < 0 || (nops > 0 && !ops) || !perm_map || !tgt_out) return fail("null argument");
+    uint64_t seen = 0;
+    for (int p = 0; p < num_qubits; ++p) {
+        if (perm_map[p] < 0 || perm_map[p] >= num_qubits || ((seen >> perm_map[p]) & 1))
+            return fail("perm_map is not a permutation");
+        seen |= 1ull << perm_map[p];
+    }
+    for (int i = 0; i < nops; ++i) {
+        const qsim_op &o = ops[i];
+        const bool two = o.kind == QSIM_OP_2Q;
+        if ((o.kind != QSIM_OP_1Q && !two && o.kind != QSIM_OP_PROBE) || o.q0 < 0 || o.q0 >= num_qubits ||
+            (two && (o.q1 < 0 || o.q1 >= num_qubits || o.q1 == o.q0)))
+            return fail("invalid op");
+    }
+    choose_layout_impl(num_qubits, cmin, dtype == QSIM_C64 ? 18 : 17, perm_map, ops, nops, tgt_out);
+    return 0;
 }
 
 int qsim_sample(qsim_state *st, uint64_t shots, uint64_t seed, uint64_t *indices_out)

@@ -182,6 +182,16 @@ int qsim_jit_compile_check(int num_qubits, int dtype, const qsim_op *ops, int no
                            const void *mats, uint64_t mat_count, int K, int cmin, int fuse,
                            const int32_t *in_map);
 
+/* Host only, no reference counterpart: the stored bit order an op list would
+ * run on (the layout change its first chunk pass folds in).  perm_map[p] is
+ * the logical bit held by stored bit p; dtype sets the low region (2^18
+ * complex64 or 2^17 complex128 amplitudes, 2 MiB); cmin is the row bits.
+ * Writes tgt_out[q] = stored position of logical bit q: the op qubits end
+ * below max(#op qubits, region), the row bits holding op qubits first; only
+ * the incoming qubits and those they displace move.  Returns 0, or -1. */
+int qsim_choose_layout(int num_qubits, int dtype, int cmin, const int32_t *perm_map, const qsim_op *ops, int nops,
+                       int32_t *tgt_out);
+
 /* No reference counterpart.  Fusion mode of forward chunk passes.  A fused
  * register step applies one 4-qubit (16x16) unitary, the complex128 product
  * of its gates rounded once, as a complex GEMM on the FP32 matrix cores

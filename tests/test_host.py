@@ -328,6 +328,38 @@ def test_chunk_search_never_worse_than_in_order(hip):
     assert fewer > 0
 
 
+@pytest.mark.parametrize("seed", range(16))
+def test_layout_choice_moves_only_incoming_qubits(hip, seed):
+    """The stored order an op list runs on: a permutation; op qubits end in
+    the low region (below max(#op qubits, 18 for complex64)); the row bits
+    hold op qubits when enough come in; op qubits already low and untouched
+    non-op qubits stay; a qubit leaving a row bit lands right above the
+    region."""
+    rng = np.random.default_rng(seed)
+    n = int(rng.integers(19, 34))
+    cmin = 4
+    perm_map = [int(x) for x in rng.permutation(n)]
+    T = sorted(int(x) for x in rng.choice(n, int(rng.integers(6, 19)), replace=False))
+    ops = [("1q", q, G.h()) for q in T]
+    arr, arena = hip.pack_ops(ops, n, np.complex64)
+    tgt = hip.choose_layout(n, perm_map, arr, len(ops), np.complex64, cmin)
+    assert sorted(tgt) == list(range(n))
+    r = min(n, max(len(T), 18))
+    cur = {perm_map[p]: p for p in range(n)}
+    assert all(tgt[q] < r for q in T)
+    incoming = [q for q in T if cur[q] >= r]
+    nonop_rows = [perm_map[p] for p in range(cmin) if perm_map[p] not in T]
+    if len(incoming) >= len(nonop_rows):
+        assert all(tgt.index(p) in T for p in range(cmin))  # every row bit holds an op qubit
+    for q in range(n):
+        if q in T and cur[q] < r:
+            assert tgt[q] == cur[q]  # op qubits already low stay
+    moved = [q for q in range(n) if tgt[q] != cur[q]]
+    assert len(moved) <= 3 * len(incoming)
+    for q in nonop_rows[:len(incoming)]:
+        assert r <= tgt[q] < r + cmin  # row leavers land just above the region
+
+
 def test_random33_tiles_plan_in_two_passes(hip):
     """Every tile of the 33-qubit depth-30 program (18 qubits each, so two
     13-bit chunks at least) plans in exactly two passes."""
