@@ -360,6 +360,17 @@ def test_layout_choice_moves_only_incoming_qubits(hip, seed):
         assert r <= tgt[q] < r + cmin  # row leavers land just above the region
 
 
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py measures one GPU: --gpus 2 exits non-zero with a message and
+    touches no device."""
+    import subprocess
+    import sys
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120, env=dict(os.environ, HIP_VISIBLE_DEVICES=""))
+    assert r.returncode != 0
+    assert "one GPU" in (r.stdout + r.stderr)
+
+
 def test_random33_tiles_plan_in_two_passes(hip):
     """Every tile of the 33-qubit depth-30 program (18 qubits each, so two
     13-bit chunks at least) plans in exactly two passes."""
