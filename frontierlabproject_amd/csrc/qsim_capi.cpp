@@ -412,8 +412,12 @@ int jit_for(qsim_state *st, const Plan &plan, bool dbl, int nv, std::vector<hipF
 
 // A stored order that differs from the logical one is kept across op lists
 // when their chunk passes run specialised kernels (a layout change then folds
-// into a first pass); otherwise it is applied first.
-bool can_fold(const qsim_state *st) { return st->perm_pending && st->jit; }
+// into a first pass; complex64 folds read amplitude pairs, so they need at
+// least one row bit); otherwise it is applied first.
+bool can_fold(const qsim_state *st)
+{
+    return st->perm_pending && st->jit && (st->dtype == QSIM_C128 || st->cmin >= 1);
+}
 
 // Stored bits below low_region() span one 2 MiB window of amplitudes (2^18
 // complex64, 2^17 complex128): chunk passes over them gather and scatter

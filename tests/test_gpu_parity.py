@@ -623,6 +623,31 @@ def test_stored_layout_moves_op_qubits_low(hip, mode, dtype):
         assert c["graph_launches"] > c["graph_captures"]
 
 
+@pytest.mark.parametrize("cmin", [0, 1])
+def test_stored_layout_with_few_row_bits(hip, cmin):
+    """complex64 with no row bits cannot fold a layout change (its reads move
+    amplitude pairs): the stored order is then applied first; with one row bit
+    it folds.  Both match the oracle."""
+    n = 20
+    rng = np.random.default_rng(71)
+    L = _layouts(n, rng, 1)
+    m1 = hip.permute_map(L[0], L[1])
+    T = [m1[18], m1[19]] + [int(q) for q in rng.permutation(n) if q not in (m1[18], m1[19])][:8]
+    ops = [(o[0], *[T[q] for q in o[1:-1]], o[-1]) for o in _tuples(C.random_circuit(len(T), 3, 72))]
+    psi0 = C.random_state(n, 13)
+    ref = P.permute(psi0, L[0], L[1])
+    for o in ops:
+        ref = dense.apply(ref, o[-1], list(o[1:-1]), n)
+    sv = hip.StatevectorF32(n)
+    sv.set_tile_params(12, cmin)
+    sv.from_numpy(psi0)
+    sv.permute(L[0], L[1])
+    sv.apply_tile_cooperative(ops)
+    _check(ref, sv.to_numpy(), np.complex64)
+    c = sv.counters()
+    assert (c["folded_permutations"] >= 1) if cmin else (c["folded_permutations"] == 0 and c["permutations"] >= 1)
+
+
 @pytest.mark.parametrize("dtype", [np.complex64, np.complex128])
 @pytest.mark.parametrize("jit", [True, False])
 @pytest.mark.parametrize("mode", ["apply_tile_cooperative", "apply_tile_graphed", "apply_tile_dev"])
