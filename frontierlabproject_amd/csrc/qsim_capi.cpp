@@ -494,10 +494,14 @@ int plan_and_compile(qsim_state *st, const qsim_op *ops, int nops, const void *m
     after.pending = false;
     if (!keep) {
         if (plan_for(st, ops, nops, mats, 1, plan)) return -1;
-        // a pending |0...0> is generated by a specialised first pass
+        // a pending |0...0> is generated by a specialised first pass; forward
+        // lists carry no probes, so with specialised kernels on, every pass has
+        // one.  A graph captured from this plan is keyed on the pending |0...0>,
+        // so it must never rely on an eager write made outside the capture.
         plan.zero_input = st->zero_pending && st->jit;
         if (jit_for(st, plan, sizeof(R) == 8, 1, fns)) return -1;
-        if (plan.zero_input && (fns.empty() || !fns[0])) plan.zero_input = false;
+        if (plan.zero_input && (fns.empty() || !fns[0]))
+            return fail("internal: generated first pass without its kernel");
         if (st->zero_pending && !plan.zero_input && materialize_zero(st)) return -1;
         return 0;
     }
@@ -588,7 +592,6 @@ int run_graphed(qsim_state *st, const qsim_op *ops, int nops, const void *mats, 
             std::memcpy(st->perm_map, g.after_map, sizeof(int32_t) * st->n);
             st->zero_pending = false;
             st->counters[1]++;
-            st->counters[3] += 0;
             return 0;
         }
     }

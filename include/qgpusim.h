@@ -232,6 +232,8 @@ int qsim_profile_read(qsim_state *st, double *ms_out, int64_t *count_out, int nc
 /* Counters: [0] kernel launches, [1] graph launches, [2] graph captures,
  * [3] chunk passes, [4] permutation passes run, [5] chunk passes run by
  * specialised kernels, [6] layout changes folded into a chunk pass.
+ * Launches are counted when enqueued or captured: a graph replay adds to [1]
+ * (and to [6] when it replays a folded layout change) only.
  * Returns the number written. */
 int qsim_get_counters(const qsim_state *st, int64_t *out, int n);
 
