@@ -44,7 +44,10 @@ class BitLocations:
 
 
 class Instruction:
-    """Base of every operation: ``name``, ``num_qubits``, ``params``."""
+    """Base of every operation: ``name``, ``num_qubits``, ``params``.
+    Directives (barriers, the save marker) take no layer in ``depth()``."""
+
+    _directive = False
 
     def __init__(self, name, num_qubits, params=()):
         self.name = name
@@ -90,7 +93,10 @@ class UnitaryGate(Gate):
 
 
 class SaveStatevector(Instruction):
-    """Terminal all-qubit marker without a matrix (skipped by the backend)."""
+    """Terminal all-qubit marker without a matrix (skipped by the backend).
+    A directive, as Aer's save instructions are."""
+
+    _directive = True
 
     def __init__(self, num_qubits):
         super().__init__("save_statevector", num_qubits)
@@ -104,6 +110,8 @@ class Measure(Instruction):
 
 
 class Barrier(Instruction):
+    _directive = True
+
     def __init__(self, num_qubits):
         super().__init__("barrier", num_qubits)
 
@@ -321,14 +329,17 @@ class QuantumCircuit:
 
     def depth(self):
         """Layers of the circuit: each instruction sits one layer above the
-        latest earlier instruction sharing a qubit or classical bit with it;
-        barriers take no layer (Qiskit's default)."""
+        latest earlier instruction sharing a qubit or classical bit with it.
+        Directives (barriers, the save marker) take no layer but align the
+        wires they span (Qiskit's default filter).  The reference's published
+        rows rely on this: depth_t = 48 for its 16-layer random circuits with
+        the save marker present (results/result.csv:6,13)."""
         level = {}
         d = 0
         for ci in self.data:
             wires = [("q", self._index[q]) for q in ci.qubits] + [("c", c) for c in ci.clbits]
             top = max((level.get(w, 0) for w in wires), default=0)
-            if ci.operation.name == "barrier":
+            if getattr(ci.operation, "_directive", False):
                 for w in wires:
                     level[w] = top
                 continue

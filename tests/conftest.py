@@ -1,3 +1,4 @@
+import json
 import os
 import sys
 
@@ -17,3 +18,24 @@ def hip():
     """The HIP binding (import fails loudly if the library is not built)."""
     from frontierlabproject_amd.backends.custom.native import qgpusim_hip
     return qgpusim_hip
+
+
+def result_csv_rows():
+    """Known answers of the reference's published tile-path runs
+    (tests/golden/result_csv_custom.json, extracted from its results/result.csv
+    by tests/golden/make_result_csv_golden.py)."""
+    with open(os.path.join(ROOT, "tests", "golden", "result_csv_custom.json")) as f:
+        return json.load(f)["rows"]
+
+
+def reference_random_twin(n, depth):
+    """The gate-structure twin of the reference's transpiled random circuit
+    (oracle.circuits.reference_random_circuit) with its save marker, as a
+    product circuit."""
+    from frontierlabproject_amd.circuit import QuantumCircuit
+    from oracle.circuits import reference_random_circuit
+    qc = QuantumCircuit(n)
+    for o in reference_random_circuit(n, depth):
+        getattr(qc, o.name)(*o.qubits)
+    qc.save_statevector()
+    return qc

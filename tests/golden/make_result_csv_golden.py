@@ -1,0 +1,50 @@
+"""Writes result_csv_custom.json: the known answers of the reference's
+published tile-path runs, taken from the data file it ships
+(qgpusim results/result.csv; columns as written by run_benchmarks.py:551-566
+and runner.py:136-174).
+
+Only the rows of the custom (LQR tile) backend are kept, one per distinct set
+of answers: circuit shape after transpile (nqubits_t, depth_t, total_gates,
+twoq_count, cx_count) and the metrics run_custom_backend returned
+(backend.py:505-520).  Timing columns are dropped.
+
+Run once where the reference is present:
+    python tests/golden/make_result_csv_golden.py /root/reference/results/result.csv
+"""
+import csv
+import json
+import os
+import sys
+
+INT_FIELDS = ("nqubits", "depth", "nL", "nqubits_t", "depth_t", "total_gates", "twoq_count", "cx_count",
+              "num_tiles", "num_gates_custom", "num_fused_ops", "num_reorders", "num_kernel_launches",
+              "num_graph_replays", "avoided_launches")
+FLOAT_FIELDS = ("fusion_ratio", "t_launch_overhead_saved_us")
+STR_FIELDS = ("circuit", "task", "tile_mode", "backend")
+
+
+def main(src, dst):
+    rows = []
+    seen = set()
+    with open(src, newline="") as f:
+        for line, r in enumerate(csv.DictReader(f), start=2):
+            if r.get("backend") != "custom" or not r.get("num_tiles"):
+                continue
+            rec = {k: int(r[k]) for k in INT_FIELDS}
+            rec.update({k: float(r[k]) for k in FLOAT_FIELDS})
+            rec.update({k: r[k] for k in STR_FIELDS})
+            key = json.dumps(rec, sort_keys=True)
+            if key in seen:
+                continue
+            seen.add(key)
+            rec["csv_line"] = line
+            rows.append(rec)
+    with open(dst, "w") as f:
+        json.dump({"source": "results/result.csv", "rows": rows}, f, indent=1, sort_keys=True)
+        f.write("\n")
+    print(f"{len(rows)} rows -> {dst}")
+
+
+if __name__ == "__main__":
+    here = os.path.dirname(os.path.abspath(__file__))
+    main(sys.argv[1], os.path.join(here, "result_csv_custom.json"))

@@ -9,6 +9,8 @@ from oracle import dense, gates as G, lqr
 from oracle import permute as P
 from oracle import svref
 
+from conftest import result_csv_rows
+
 
 def test_gate_identities():
     I2 = np.eye(2)
@@ -101,12 +103,25 @@ def test_permute_errors(lo, ln, msg):
         P.perm_map(lo, ln)
 
 
-@pytest.mark.parametrize("n,tiles,reorders,ops", [(20, 5, 4, 624), (25, 9, 8, 784)])
-def test_lqr_reference_known_answers(n, tiles, reorders, ops):
-    """results/result.csv rows 6 and 13: random d=16 nL=16 custom runs."""
-    circ = C.reference_random_circuit(n, 16) + [C.marker(n)]
-    plan, _, _ = lqr.lqr(circ, n, 16)
-    assert lqr.summarize(plan) == {"num_tiles": tiles, "num_reorders": reorders, "num_gates": ops}
+@pytest.mark.parametrize("row", result_csv_rows(), ids=lambda r: f"csv_line{r['csv_line']}")
+def test_lqr_reference_known_answers(row):
+    """The reference's published custom-backend rows (results/result.csv):
+    the gate-structure twin of its transpiled random circuit has the row's
+    shape (qubits, gates, CX, depth with the save marker taking no layer),
+    and the schedule restatement gives the row's tiles, reorders and ops."""
+    n, d = row["nqubits"], row["depth"]
+    body = C.reference_random_circuit(n, d)
+    circ = body + [C.marker(n)]
+    level = [0] * n
+    for g in body:
+        top = max(level[q] for q in g.qubits) + 1
+        for q in g.qubits:
+            level[q] = top
+    assert (n, len(body), max(level)) == (row["nqubits_t"], row["total_gates"], row["depth_t"])
+    assert sum(g.name == "cx" for g in body) == row["cx_count"] == row["twoq_count"]
+    plan, _, _ = lqr.lqr(circ, n, row["nL"])
+    assert lqr.summarize(plan) == {"num_tiles": row["num_tiles"], "num_reorders": row["num_reorders"],
+                                   "num_gates": row["num_gates_custom"]}
 
 
 @pytest.mark.parametrize("seed", range(6))
