@@ -165,6 +165,7 @@ int qsim_batch_load(qsim_batch *b, int ncirc, const int32_t *num_qubits, const u
         const int n = num_qubits[i];
         if (n < 6 || n > 34) return set_error("batch: qubit counts must be in [6, 34]");
         if (nprobes[i] < 0) return set_error("batch: negative probe count");
+        if (fwd_off[i] < 0 || rev_off[i] < 0) return set_error("batch: op offsets must be non-negative");
         const int64_t nf = fwd_off[i + 1] - fwd_off[i], nr = rev_off[i + 1] - rev_off[i];
         if (nf < 0 || nr < 0) return set_error("batch: op offsets must be non-decreasing");
         const qsim_op *fo = fwd_ops + fwd_off[i], *ro = rev_ops + rev_off[i];
@@ -277,7 +278,15 @@ int qsim_batch_run(qsim_batch *b, int use_graph, double *expect_out, double *pro
                    double *total_ms_out)
 {
     if (!b || !b->d_desc) return set_error("batch: nothing loaded");
-    hipEvent_t e0 = nullptr, e1 = nullptr;
+    struct Events {  // destroyed on every return
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        ~Events()
+        {
+            if (e0) hipEventDestroy(e0);
+            if (e1) hipEventDestroy(e1);
+        }
+    } ev;
+    hipEvent_t &e0 = ev.e0, &e1 = ev.e1;
     HIP_OK(hipEventCreate(&e0));
     HIP_OK(hipEventCreate(&e1));
     HIP_OK(hipEventRecord(e0, b->stream));
@@ -313,8 +322,6 @@ int qsim_batch_run(qsim_batch *b, int use_graph, double *expect_out, double *pro
     HIP_OK(hipStreamSynchronize(b->stream));
     float ms = 0.f;
     HIP_OK(hipEventElapsedTime(&ms, e0, e1));
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
     if (total_ms_out) *total_ms_out = ms;
     if (exec_ms_out)
         for (int i = 0; i < b->ncirc; ++i)

@@ -1261,7 +1261,8 @@ int qsim_sample(qsim_state *st, uint64_t shots, uint64_t seed, uint64_t *indices
         z ^= z >> 31;
         const double u = (double)(z >> 11) * 0x1p-53 * total;
         uint64_t b = (uint64_t)(std::upper_bound(cdf.begin(), cdf.end(), u) - cdf.begin());
-        if (b >= nb) b = nb - 1;
+        // u rounded up to the total: the last block with nonzero probability
+        if (b >= nb) b = (uint64_t)(std::lower_bound(cdf.begin(), cdf.end(), total) - cdf.begin());
         draws[k] = {b, u - (b ? cdf[b - 1] : 0.0)};
     }
     std::sort(draws.begin(), draws.end());

@@ -65,12 +65,16 @@ __global__ __launch_bounds__(kTPB) void k_sample_blocks(const cx<R> *__restrict_
     const double before = threadIdx.x ? part[threadIdx.x - 1] : 0.0;
     for (int k = 0; k < PER; ++k) cdf[e0 + k] += before;
     __syncthreads();
+    const double top = cdf[len - 1];
     for (uint32_t s = threadIdx.x; s < g.count; s += kTPB) {
         const double u = residual[g.first + s];
-        int lo = 0, hi = len - 1;  // first e with cdf[e] > u (the last one if rounding leaves none)
+        // first e with cdf[e] > u; when rounding leaves none, the first e with
+        // cdf[e] == top, the last amplitude with nonzero probability
+        const bool over = !(u < top);
+        int lo = 0, hi = len - 1;
         while (lo < hi) {
             const int mid = (lo + hi) >> 1;
-            if (cdf[mid] > u) hi = mid;
+            if (over ? cdf[mid] >= top : cdf[mid] > u) hi = mid;
             else lo = mid + 1;
         }
         index_out[g.first + s] = b0 + (uint64_t)lo;
