@@ -463,6 +463,8 @@ def jit_compile_check(num_qubits, arr, nops, arena, K, cmin, fuse=False, in_map=
     (Statevector.set_fusion) or a bool for mode 1."""
     code = C128 if arena.dtype == np.complex128 else C64
     m = None if in_map is None else np.ascontiguousarray(in_map, dtype=np.int32)
+    if m is not None and m.size != num_qubits:
+        raise ValueError("in_map must have num_qubits entries")
     npass = lib.qsim_jit_compile_check(num_qubits, code, ctypes.addressof(arr), nops, arena.ctypes.data,
                                        arena.size, K, cmin, int(fuse),
                                        None if m is None else m.ctypes.data)
@@ -477,6 +479,8 @@ def choose_layout(num_qubits, perm_map, arr, nops, dtype=np.complex64, cmin=None
     code = C128 if dtype == np.complex128 else C64
     cm = (4 if code == C64 else 3) if cmin is None else cmin
     pm = np.ascontiguousarray(perm_map, dtype=np.int32)
+    if pm.size != num_qubits:
+        raise ValueError("perm_map must have num_qubits entries")
     out = np.zeros(num_qubits, dtype=np.int32)
     _check(lib.qsim_choose_layout(num_qubits, code, cm, pm.ctypes.data, ctypes.addressof(arr), nops,
                                   out.ctypes.data))

@@ -353,6 +353,23 @@ def test_chunk_search_never_worse_than_in_order(hip):
     assert fewer > 0
 
 
+def test_layout_choice_refuses_bad_input(hip):
+    """Maps of the wrong length, maps that are not permutations and invalid
+    ops are refused before anything is read past them."""
+    n = 20
+    arr, _ = hip.pack_ops([("1q", 3, G.h())], n, np.complex64)
+    with pytest.raises(ValueError, match="num_qubits entries"):
+        hip.choose_layout(n, list(range(n - 1)), arr, 1)
+    with pytest.raises(RuntimeError, match="perm_map is not a permutation"):
+        hip.choose_layout(n, [0] * n, arr, 1)
+    arr[0].q0 = n
+    with pytest.raises(RuntimeError, match="invalid op"):
+        hip.choose_layout(n, list(range(n)), arr, 1)
+    arr, arena = hip.pack_ops([("1q", 3, G.h())], n, np.complex64)
+    with pytest.raises(ValueError, match="num_qubits entries"):
+        hip.jit_compile_check(n, arr, 1, arena, 13, 4, in_map=list(range(n - 2)))
+
+
 @pytest.mark.parametrize("seed", range(16))
 def test_layout_choice_moves_only_incoming_qubits(hip, seed):
     """The stored order an op list runs on: a permutation; op qubits end in
