@@ -130,6 +130,19 @@ def _matrix(U, k, dtype):
     return np.ascontiguousarray(m, dtype=dtype)
 
 
+def _check_packed(arr, nops, arena=None, dtype=None):
+    """A packed op array and matrix arena as the library reads them: at most
+    len(arr) ops; the arena contiguous, complex, and in the state's precision
+    when one is given (the library reads matrices in that precision)."""
+    if nops < 0 or nops > len(arr):
+        raise ValueError("nops exceeds the op array")
+    if arena is not None:
+        if arena.dtype not in (np.complex64, np.complex128) or not arena.flags.c_contiguous:
+            raise ValueError("matrix arena must be a contiguous complex64 or complex128 array")
+        if dtype is not None and arena.dtype != np.dtype(dtype):
+            raise ValueError(f"matrix arena must be {np.dtype(dtype).name} for this state")
+
+
 def pack_ops(ops, num_qubits, dtype):
     """Parse ('1q', q0, U) / ('2q', q0, q1, U) tuples (bindings.cpp:182-220)
     into a ctypes op array and one contiguous matrix arena."""
@@ -268,6 +281,7 @@ class _StatevectorBase:
 
     def apply_packed(self, mode, arr, nops, arena):
         """Apply a pre-packed op array (see pack_ops) without re-parsing."""
+        _check_packed(arr, nops, arena, self._DTYPE)
         _check(lib.qsim_apply_ops(self._h, mode, ctypes.addressof(arr), nops,
                                   arena.ctypes.data, arena.size))
 
@@ -322,6 +336,7 @@ class _StatevectorBase:
 
     def adjoint(self, terms, rev_ops, nops, arena, nprobes):
         """Run the adjoint reverse sweep (see qsim_adjoint); returns (probes, <H>)."""
+        _check_packed(rev_ops, nops, arena, self._DTYPE)
         arr, nt = pauli_array(terms)
         probes = np.zeros(max(1, nprobes), dtype=np.float64)
         e = _D()
@@ -461,6 +476,7 @@ def jit_compile_check(num_qubits, arr, nops, arena, K, cmin, fuse=False, in_map=
     -> in_map[p]) and compile every specialised pass kernel for gfx950 without
     loading it; returns the number of passes.  fuse: a fusion mode
     (Statevector.set_fusion) or a bool for mode 1."""
+    _check_packed(arr, nops, arena)
     code = C128 if arena.dtype == np.complex128 else C64
     m = None if in_map is None else np.ascontiguousarray(in_map, dtype=np.int32)
     if m is not None and m.size != num_qubits:
@@ -476,6 +492,7 @@ def jit_compile_check(num_qubits, arr, nops, arena, K, cmin, fuse=False, in_map=
 def choose_layout(num_qubits, perm_map, arr, nops, dtype=np.complex64, cmin=None):
     """Host-only: stored position of each logical bit for an op list run on
     the stored order perm_map (perm_map[p] = logical bit at stored bit p)."""
+    _check_packed(arr, nops)
     code = C128 if dtype == np.complex128 else C64
     cm = (4 if code == C64 else 3) if cmin is None else cmin
     pm = np.ascontiguousarray(perm_map, dtype=np.int32)
@@ -490,6 +507,7 @@ def choose_layout(num_qubits, perm_map, arr, nops, dtype=np.complex64, cmin=None
 def plan_describe(num_qubits, arr, nops, arena, K, cmin, steps=False):
     """Host-only chunk plan: ([(K, cfast, nsteps, pos[, steps])], execution order),
     a step being (register bits, rounds, dense)."""
+    _check_packed(arr, nops, arena)
     code = C128 if arena.dtype == np.complex128 else C64
     cap = 4096 + 128 * max(1, nops)
     desc = np.zeros(cap, dtype=np.int32)

@@ -83,10 +83,13 @@ def test_lqr_pass_matches_oracle(seed):
         [idx_p[id(nd.op)] for nd in lqr_pass.property_set["flat_tiling_schedule"]]
 
 
-@pytest.mark.parametrize("n,tiles,reorders", [(20, 5, 4), (25, 9, 8)])
-def test_lqr_pass_reference_known_answers(n, tiles, reorders):
+@pytest.mark.parametrize("row", result_csv_rows(), ids=lambda r: f"csv_line{r['csv_line']}")
+def test_lqr_pass_reference_known_answers(row):
+    """The published rows with general 1q gates (u) in place of the merged
+    rotations: the schedule depends only on which qubits each gate touches."""
+    n, tiles, reorders = row["nqubits"], row["num_tiles"], row["num_reorders"]
     qc = qcirc.QuantumCircuit(n)
-    for _ in range(16):
+    for _ in range(row["depth"]):
         for q in range(n):
             qc.u(0.1, 0.2, 0.3, q)
         for i in range(0, n - 1, 2):
@@ -94,7 +97,7 @@ def test_lqr_pass_reference_known_answers(n, tiles, reorders):
         for i in range(1, n - 1, 2):
             qc.cx(i, i + 1)
     qc.save_statevector()
-    pm, p = make_custom_pm(16)
+    pm, p = make_custom_pm(row["nL"])
     pm.run(qc)
     plan = p.property_set["flat_tiling_plan"]
     layout, reord = list(range(n)), 0
@@ -103,7 +106,7 @@ def test_lqr_pass_reference_known_answers(n, tiles, reorders):
             reord += 1
             layout = t.layout
     assert (len(plan), reord) == (tiles, reorders)
-    assert sum(1 for t in plan for g in t.gates if hasattr(g.op, "to_matrix")) == 16 * (2 * n - 1)
+    assert sum(1 for t in plan for g in t.gates if hasattr(g.op, "to_matrix")) == row["num_gates_custom"]
 
 
 def test_summarize_tiling():
@@ -353,7 +356,7 @@ def test_chunk_search_never_worse_than_in_order(hip):
     assert fewer > 0
 
 
-def test_layout_choice_refuses_bad_input(hip):
+def test_host_helpers_refuse_bad_input(hip):
     """Maps of the wrong length, maps that are not permutations and invalid
     ops are refused before anything is read past them."""
     n = 20
@@ -368,6 +371,14 @@ def test_layout_choice_refuses_bad_input(hip):
     arr, arena = hip.pack_ops([("1q", 3, G.h())], n, np.complex64)
     with pytest.raises(ValueError, match="num_qubits entries"):
         hip.jit_compile_check(n, arr, 1, arena, 13, 4, in_map=list(range(n - 2)))
+    # packed ops: no count past the array, no arena the library would misread
+    with pytest.raises(ValueError, match="exceeds the op array"):
+        hip.plan_describe(n, arr, 2, arena, 13, 4)
+    with pytest.raises(ValueError, match="contiguous complex64 or complex128"):
+        hip.plan_describe(n, arr, 1, arena.view(np.float32), 13, 4)
+    with pytest.raises(ValueError, match="must be complex128 for this state"):
+        hip._check_packed(arr, 1, arena, np.complex128)
+    hip._check_packed(arr, 1, arena, np.complex64)
 
 
 @pytest.mark.parametrize("seed", range(16))
