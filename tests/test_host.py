@@ -135,6 +135,40 @@ def test_gate_matrices_match_oracle():
         assert np.allclose(g.inverse().to_matrix() @ g.to_matrix(), np.eye(2 ** k))
 
 
+def test_remaining_gate_matrices_pinned():
+    """Gates the oracle has no entry for, pinned by identities over its closed
+    forms (little-endian: kron(M_q1, M_q0) acts on index b_q0 + 2 b_q1)."""
+    m = lambda name, *p: qcirc.gate(name, *p).to_matrix()
+    I2 = np.eye(2)
+    assert np.allclose(m("id"), I2)
+    assert np.allclose(m("sdg"), G.s().conj().T)
+    assert np.allclose(m("tdg"), G.t().conj().T)
+    assert np.allclose(m("sx") @ m("sx"), G.x())
+    assert np.allclose(m("sx"), np.exp(1j * np.pi / 4) * G.rx(np.pi / 2))
+    for th in (0.0, 0.7, -2.3):
+        assert np.allclose(m("u", th, -np.pi / 2, np.pi / 2), G.rx(th))
+        assert np.allclose(m("u", th, 0.0, 0.0), G.ry(th))
+        assert np.allclose(m("u", 0.0, 0.0, th), G.p(th))
+    assert np.allclose(m("u", np.pi / 2, 0.0, np.pi), G.h())
+    # u(theta, phi, lam) = P(phi) RY(theta) P(lam)
+    assert np.allclose(m("u", 0.4, 1.1, -0.6), G.p(1.1) @ G.ry(0.4) @ G.p(-0.6))
+    sdg = G.s().conj().T
+    assert np.allclose(m("cy"), np.kron(G.s(), I2) @ G.cx() @ np.kron(sdg, I2))
+    P0, P1 = np.diag([1.0, 0.0]), np.diag([0.0, 1.0])
+    assert np.allclose(m("crz", 0.9), np.kron(I2, P0) + np.kron(G.rz(0.9), P1))
+    rng = np.random.default_rng(2)
+    for name in qcirc._MATRIX:
+        g = qcirc.gate(name, *rng.uniform(-3, 3, len(_param_names(name))))
+        k = g.num_qubits
+        assert np.allclose(g.inverse().to_matrix() @ g.to_matrix(), np.eye(2 ** k)), name
+
+
+def _param_names(name):
+    """Parameter names of a gate's matrix function."""
+    import inspect
+    return list(inspect.signature(qcirc._MATRIX[name]).parameters)
+
+
 def test_generators_match_oracle():
     qc = W.random_circuit(12, 7, 77)
     ops = C.random_circuit(12, 7, 77)
