@@ -4,6 +4,8 @@ Tolerances (from the task's precision targets): 1 - fidelity <= 1e-10 for
 complex128 and <= 1e-5 for complex64; permutations are bit-exact (pure data
 movement).  Sizes are chosen so the oracle finishes in seconds.
 """
+import re
+
 import numpy as np
 import pytest
 
@@ -519,6 +521,49 @@ def test_batch_matches_single_circuit_runs():
             z = np.where(np.arange(1 << n) & 1, -1.0, 1.0)
             assert abs(e[i] - float(np.sum(z * np.abs(psi) ** 2))) < 2e-5
     sim.close()
+
+
+@pytest.mark.parametrize("dtype", [np.complex64, np.complex128])
+def test_state_api_errors_match_reference(hip, dtype):
+    """Each refused call raises RuntimeError with the reference's message
+    (bindings.cpp:191-216, 437-462, 503-533, 567) and leaves the state, a
+    pending layout change included, as it was."""
+    n = 6
+    sv = _sv(hip, n, dtype)
+    psi0 = C.random_state(n, 9)
+    sv.from_numpy(psi0)
+    ident = list(range(n))
+    L = list(reversed(ident))
+    sv.permute(ident, L)
+    U2, U4 = G.h(), G.cx()
+    cases = [
+        (lambda: sv.from_numpy(np.zeros(3, dtype=dtype)), "Input array must have size 2^num_qubits"),
+        (lambda: sv.apply_1q(n, U2), "Invalid target qubit"),
+        (lambda: sv.apply_1q(0, U4), "U must be a 2x2 matrix"),
+        (lambda: sv.apply_2q(0, n, U4), "Invalid qubit indices"),
+        (lambda: sv.apply_2q(1, 1, U4), "q0 and q1 must be different"),
+        (lambda: sv.apply_2q(0, 1, U2), "U must be a 4x4 matrix"),
+        (lambda: sv.permute(ident[:-1], ident), "layout_old/layout_new must have length num_qubits"),
+        (lambda: sv.permute(L, [0, 1, 2, 3, 4, n]), "layout_new contains invalid qubit id"),
+        (lambda: sv.permute(L, [0, 0, 2, 3, 4, 5]), "layout_new is not a permutation"),
+        (lambda: sv.permute([0, 1, 2, 3, 4, -1], ident), "layout_old contains invalid qubit id"),
+        (lambda: sv.apply_tile_cooperative([()]), "apply_tile_dev: empty tuple op descriptor"),
+        (lambda: sv.apply_tile_cooperative([("1q", 0)]), "apply_tile_dev: 1q expects ('1q', q0, U_dev)"),
+        (lambda: sv.apply_tile_graphed([("1q", n, U2)]), "apply_tile_dev: invalid q0"),
+        (lambda: sv.apply_tile_dev([("2q", 0, 1)]), "apply_tile_dev: 2q expects ('2q', q0, q1, U_dev)"),
+        (lambda: sv.apply_tile_cooperative([("2q", 0, n, U4)]), "apply_tile_dev: invalid q0/q1"),
+        (lambda: sv.apply_tile_cooperative([("2q", 2, 2, U4)]), "apply_tile_dev: q0 == q1"),
+        (lambda: sv.apply_tile_cooperative([("3q", 0, 1, 2, U4)]), "apply_tile_dev: unknown op kind: 3q"),
+    ]
+    for call, msg in cases:
+        with pytest.raises(RuntimeError, match=re.escape(msg)):
+            call()
+    # the library's own check of a packed list that bypassed the Python parser
+    arr, arena = hip.pack_ops([("1q", 0, U2)], n, dtype)
+    arr[0].q0 = n
+    with pytest.raises(RuntimeError, match="apply_tile_dev: invalid q0"):
+        sv.apply_packed(hip.COOPERATIVE, arr, 1, arena)
+    assert np.array_equal(sv.to_numpy(), P.permute(psi0.astype(dtype), ident, L))
 
 
 def test_batch_rejects_bad_offsets():
