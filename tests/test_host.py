@@ -109,15 +109,26 @@ def test_lqr_pass_reference_known_answers(row):
     assert sum(1 for t in plan for g in t.gates if hasattr(g.op, "to_matrix")) == row["num_gates_custom"]
 
 
-def test_summarize_tiling():
-    qc = W.random_circuit(8, 4, 1)
-    pm, p = make_custom_pm(4)
-    pm.run(qc)
+@pytest.mark.parametrize("seed", range(6))
+def test_summarize_tiling(seed):
+    """summarize_tiling (passes.py:274-313): every field equals the statistic
+    taken from the oracle's tile plan of the same circuit; empty schedule."""
+    rng = np.random.default_rng(seed)
+    n = int(rng.integers(4, 12))
+    nL = int(rng.integers(2, n + 1))
+    ops = C.random_circuit(n, int(rng.integers(1, 7)), 900 + seed)
+    oplan, _, _ = lqr.lqr(ops, n, nL)
+    pm, p = make_custom_pm(nL)
+    pm.run(_to_product(ops, n))
     s = LazyQubitReordering.summarize_tiling(p.property_set["flat_tiling_schedule"],
                                              {id(k): sum(1 << q for q in v) for k, v in
                                               p.property_set["flat_tiling_mapping"].items()})
-    assert s["num_tiles"] == len(p.property_set["flat_tiling_plan"])
-    assert s["max_Q_L_size"] <= 4
+    sizes = [len(t.gates) for t in oplan]
+    qls = [len(t.logical_qubits) for t in oplan]
+    assert s == {"num_tiles": len(oplan), "avg_tile_size": sum(sizes) / len(oplan), "max_tile_size": max(sizes),
+                 "avg_Q_L_size": sum(qls) / len(oplan), "max_Q_L_size": max(qls)}
+    assert LazyQubitReordering.summarize_tiling([], {}) == {
+        "num_tiles": 0, "avg_tile_size": 0.0, "max_tile_size": 0, "avg_Q_L_size": 0.0, "max_Q_L_size": 0}
 
 
 def test_gate_matrices_match_oracle():
