@@ -115,7 +115,7 @@ def result_row(combo: Dict[str, object], t_trans: float, t_sim: float, stats: Di
 def log_run(csv_path: str, qc, combo: Dict[str, object], reference=None, repeat_idx: int = 1, sample_seed: int = 0):
     """Run ``qc`` with ``runner.run_once`` (custom transpiler and backend) and
     append its row.  ``reference``: a statevector (statevector task) or a
-    count dictionary (sampling task) to judge the result against, or None."""
+    count dictionary (sampling task) to check the result against, or None."""
     from .runner import run_once
     task = combo["task"]
     t_tr, t_sim, extra, stats, usage = run_once(None, qc, task, combo.get("shots"), task == "sampling",

@@ -881,7 +881,7 @@ def test_run_once_sampling_counts():
 
 
 def test_log_run_rows(tmp_path):
-    """results.log_run: one row per run in the reference's schema, judged
+    """results.log_run: one row per run in the reference's schema, checked
     against the FP64 oracle (statevector) and oracle probabilities (sampling)."""
     import csv as _csv
     from frontierlabproject_amd import results as Rz
