@@ -180,7 +180,7 @@ def main():
     args = parse()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
-        print("bench.py: this build measures one GPU only (the statevector path runs on a "
+        print("bench.py: measures one GPU only (the statevector path runs on a "
               "single MI355X); run it with --gpus 1", file=sys.stderr)
         sys.exit(2)
 

@@ -17,7 +17,7 @@ namespace qsim {
 
 namespace {
 
-// qsim_dev_types.h + qsim_device.h as one string (generated by the build).
+// qsim_dev_types.h + qsim_device.h as one string (written by build_native.py).
 #include "qsim_jit_prelude.inc"
 
 std::mutex g_mu;

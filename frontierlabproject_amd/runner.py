@@ -28,7 +28,7 @@ def run_once(sim, qc, task: str, shots: int, measure: bool, transpiler: str = "b
     Returns (transpile_s, simulate_s, extra, stats, res_usage)."""
     if transpiler != "custom" or backend != "custom":
         raise RuntimeError("run_once: the Aer paths (transpiler='baseline' or backend='aer') need qiskit-aer, "
-                           "which this build does not use; pass transpiler='custom', backend='custom'")
+                           "which this package does not use; pass transpiler='custom', backend='custom'")
     if task not in ("statevector", "sampling"):
         raise ValueError(f"run_once: unknown task {task!r}")
     qc_run = qc

@@ -1,6 +1,6 @@
 """GPU parity of the HIP path against the FP64 oracle, through the C ABI.
 
-Tolerances (from the task's precision targets): 1 - fidelity <= 1e-10 for
+Tolerances: 1 - fidelity <= 1e-10 for
 complex128 and <= 1e-5 for complex64; permutations are bit-exact (pure data
 movement).  Sizes are chosen so the oracle finishes in seconds.
 """
