@@ -716,11 +716,13 @@ def test_stored_layout_moves_op_qubits_low(hip, mode, dtype):
         assert c["graph_launches"] > c["graph_captures"]
 
 
+@pytest.mark.parametrize("dtype", [np.complex64, np.complex128])
 @pytest.mark.parametrize("cmin", [0, 1])
-def test_stored_layout_with_few_row_bits(hip, cmin):
+def test_stored_layout_with_few_row_bits(hip, dtype, cmin):
     """complex64 with no row bits cannot fold a layout change (its reads move
     amplitude pairs): the stored order is then applied first; with one row bit
-    it folds.  Both match the oracle."""
+    it folds.  complex128 reads single amplitudes and folds with either.  All
+    match the oracle."""
     n = 20
     rng = np.random.default_rng(71)
     L = _layouts(n, rng, 1)
@@ -731,14 +733,15 @@ def test_stored_layout_with_few_row_bits(hip, cmin):
     ref = P.permute(psi0, L[0], L[1])
     for o in ops:
         ref = dense.apply(ref, o[-1], list(o[1:-1]), n)
-    sv = hip.StatevectorF32(n)
-    sv.set_tile_params(12, cmin)
+    sv = _sv(hip, n, dtype)
+    sv.set_tile_params(12 if dtype == np.complex64 else 11, cmin)
     sv.from_numpy(psi0)
     sv.permute(L[0], L[1])
     sv.apply_tile_cooperative(ops)
-    _check(ref, sv.to_numpy(), np.complex64)
+    _check(ref, sv.to_numpy(), dtype)
     c = sv.counters()
-    assert (c["folded_permutations"] >= 1) if cmin else (c["folded_permutations"] == 0 and c["permutations"] >= 1)
+    folds = cmin > 0 or dtype == np.complex128
+    assert (c["folded_permutations"] >= 1) if folds else (c["folded_permutations"] == 0 and c["permutations"] >= 1)
 
 
 @pytest.mark.parametrize("dtype", [np.complex64, np.complex128])
