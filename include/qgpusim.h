@@ -160,7 +160,10 @@ int qsim_adjoint(qsim_state *st, const qsim_pauli *terms, int nterms, const qsim
                  int nops, const void *mats, uint64_t mat_count, int nprobes, double *probes_out,
                  double *expect_out);
 
-/* Chunk-pass geometry: K chunk bits, cmin always-present low bits. */
+/* Chunk-pass geometry: K chunk bits, cmin always-present low bits.  Defaults:
+ * K = 13, cmin = 4 for complex64; K = 12, cmin = 3 for complex128.  Ranges:
+ * 6 <= K <= 13 (12 for complex128), 0 <= cmin <= K - 2.  Two-vector adjoint
+ * passes use at most K = 12 (complex64) or 11 (complex128). */
 int qsim_set_tile_params(qsim_state *st, int K, int cmin);
 
 /* No reference counterpart (the reference has one fixed kernel per gate).
