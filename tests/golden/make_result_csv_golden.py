@@ -6,7 +6,10 @@ and runner.py:136-174).
 Only the rows of the custom (LQR tile) backend are kept, one per distinct set
 of answers: circuit shape after transpile (nqubits_t, depth_t, total_gates,
 twoq_count, cx_count) and the metrics run_custom_backend returned
-(backend.py:505-520).  Timing columns are dropped.
+(backend.py:505-520), parsed; plus each kept row verbatim ("csv_row", every
+column as written but the four naming the machine and time) and the file's
+header, which pin the schema and the
+number formats of a result row.
 
 Run once where the reference is present:
     python tests/golden/make_result_csv_golden.py /root/reference/results/result.csv
@@ -21,13 +24,16 @@ INT_FIELDS = ("nqubits", "depth", "nL", "nqubits_t", "depth_t", "total_gates", "
               "num_graph_replays", "avoided_launches")
 FLOAT_FIELDS = ("fusion_ratio", "t_launch_overhead_saved_us")
 STR_FIELDS = ("circuit", "task", "tile_mode", "backend")
+ENV_FIELDS = ("timestamp", "host", "os", "python")  # where and when the run was made: not kept
 
 
 def main(src, dst):
     rows = []
     seen = set()
     with open(src, newline="") as f:
-        for line, r in enumerate(csv.DictReader(f), start=2):
+        reader = csv.DictReader(f)
+        header = list(reader.fieldnames)
+        for line, r in enumerate(reader, start=2):
             if r.get("backend") != "custom" or not r.get("num_tiles"):
                 continue
             rec = {k: int(r[k]) for k in INT_FIELDS}
@@ -38,9 +44,10 @@ def main(src, dst):
                 continue
             seen.add(key)
             rec["csv_line"] = line
+            rec["csv_row"] = {k: v for k, v in r.items() if k not in ENV_FIELDS}
             rows.append(rec)
     with open(dst, "w") as f:
-        json.dump({"source": "results/result.csv", "rows": rows}, f, indent=1, sort_keys=True)
+        json.dump({"source": "results/result.csv", "header": header, "rows": rows}, f, indent=1, sort_keys=True)
         f.write("\n")
     print(f"{len(rows)} rows -> {dst}")
 
