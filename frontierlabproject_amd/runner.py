@@ -4,9 +4,12 @@
 ``(transpile_s, simulate_s, extra, stats, res_usage)`` and its stats keys.  It
 runs the path this package implements: ``transpiler="custom"`` with
 ``backend="custom"`` (the save_statevector marker, the LQR pass, then
-``run_custom_backend`` in graphed mode, complex64, fusion on).  The Qiskit
-``transpile()`` call before the LQR pass (runner.py:45) is not made: Qiskit is
-not a dependency here, and the circuit reaches the LQR pass in program order.
+``run_custom_backend`` in graphed mode, complex64, fusion on).  In place of
+the Qiskit ``transpile()`` call before the LQR pass (runner.py:45) the circuit
+goes through ``merge_1q_runs``: each maximal run of single-qubit gates on a
+qubit becomes one gate, which is what that call does to the reference's
+circuits according to its published rows (gate counts, depth and tiles are
+then the same); Qiskit is not a dependency here.
 The reference's Aer paths (``transpiler="baseline"`` or ``backend="aer"``)
 need qiskit-aer and raise a RuntimeError saying so.
 
@@ -18,6 +21,7 @@ import time
 
 from .backends.custom.backend import run_custom_backend
 from .metrics.resources import _max_usage, _mem_snapshot, _tqc_stats
+from .transpiler.canonicalize import merge_1q_runs
 from .transpiler.pm import make_custom_pm
 
 
@@ -41,6 +45,8 @@ def run_once(sim, qc, task: str, shots: int, measure: bool, transpiler: str = "b
     qc_sv = qc_run.copy()
     if task == "statevector" and save_sv:
         qc_sv.save_statevector()
+    if task == "statevector":
+        qc_sv = merge_1q_runs(qc_sv)  # runner.py:45 transpiles the statevector task only
     pm, lqr_pass = make_custom_pm(num_local_qubits)
     tqc = pm.run(qc_sv)
     tiling_plan = lqr_pass.property_set.get("flat_tiling_plan", None)
