@@ -30,7 +30,10 @@ plausible mistake would fail):
                   qgpusim_cuda/src/kernels.cu:30-49, 83-136, 173-184.  Pinned
                   against ``dense`` and ``permute``.
 
-Parity of the Qiskit-only pieces (``DAGCircuit.topological_op_nodes`` order,
-``transpile`` output) is unpinned: Qiskit is not installed, and the oracle
-defines the gate order as program order.
+The reference's published rows (results/result.csv, kept as
+tests/golden/result_csv_custom.json) also fix the shape of ``transpile``'s
+output on its random circuits: one single-qubit gate per qubit and layer.
+The rest of the Qiskit-only pieces (``DAGCircuit.topological_op_nodes``
+order, the rest of ``transpile``) is unpinned: Qiskit is not installed, and
+the oracle defines the gate order as program order.
 """
