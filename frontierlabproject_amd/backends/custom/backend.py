@@ -12,6 +12,8 @@ Differences from the reference, each deliberate:
   more qubits raise ``ValueError``; the reference drops both silently
   (backend.py:448-451);
 * fused matrices are multiplied in complex128 and then cast;
+* ``qc.global_phase`` is applied (the reference never reads it): folded into
+  the matrix of the first op the circuit applies, so it costs no pass;
 * there is no CuPy fallback simulator: the HIP library is required.
 
 ``tile_mode``:
@@ -126,6 +128,24 @@ def fuse_tile_ops(tile_ops, dtype=np.complex128):
     return result
 
 
+def fold_global_phase(tile_ops, phase, dtype):
+    """``tile_ops`` with exp(i * phase) folded into the matrix of its first op
+    (product in complex128, then cast); unchanged when phase is 0 or the list
+    is empty."""
+    if not phase or not tile_ops:
+        return tile_ops
+    first = tile_ops[0]
+    U = (np.asarray(first[-1], dtype=np.complex128) * np.exp(1j * phase)).astype(dtype)
+    return [tuple(first[:-1]) + (U,)] + list(tile_ops[1:])
+
+
+def _global_phase(qc):
+    try:
+        return float(getattr(qc, "global_phase", 0.0) or 0.0)
+    except TypeError:  # an unbound parameter expression
+        raise ValueError("run_custom_backend: the circuit's global phase has unbound parameters")
+
+
 def build_tile_ops(tile: Tile, qubit_map, np_dtype):
     """Ops of one tile with local qubit indices (backend.py:431-451)."""
     ops = []
@@ -180,6 +200,7 @@ def run_custom_backend(qc, tile_plan: List[Tile], task: str, shots: int = None,
     num_kernel_launches = num_graph_replays = 0
     t_perm = t_apply = 0.0
 
+    phase = _global_phase(qc)  # applied with the first op (or to the result when nothing runs)
     sv = qgpusim_cuda.StatevectorF32(num_qubits) if use_float32 else qgpusim_cuda.Statevector(num_qubits)
     layout = list(range(num_qubits))
     for tile in tile_plan:
@@ -195,6 +216,7 @@ def run_custom_backend(qc, tile_plan: List[Tile], task: str, shots: int = None,
         if tile_ops:
             if enable_fusion:
                 tile_ops = fuse_tile_ops(tile_ops, np_dtype)
+            tile_ops, phase = fold_global_phase(tile_ops, phase, np_dtype), 0.0
             num_fused_ops += len(tile_ops)
             if tile_mode == "cooperative":
                 sv.apply_tile_cooperative(tile_ops)
@@ -217,6 +239,8 @@ def run_custom_backend(qc, tile_plan: List[Tile], task: str, shots: int = None,
         nbits = max(getattr(qc, "num_clbits", 0) or 0, 1 + max(c for _, c in meas))
         counts = counts_from_indices(sv.sample(int(shots), seed=sample_seed), layout, meas, nbits)
     result = sv.to_numpy()
+    if phase:
+        result *= np.exp(1j * phase).astype(np_dtype)
     counters = sv.counters()
     del sv
 

@@ -324,6 +324,80 @@ def test_gate_cache_keys_dtype():
     assert a.dtype == np.complex64 and b.dtype == np.complex128
 
 
+class _OracleState:
+    """Stand-in for the native state class, on the FP64 oracle: lets the host
+    driver (tile loop, local indices, fusion, phase) run without a GPU."""
+
+    def __init__(self, n):
+        self.n = n
+        self.psi = dense.zero_state(n)
+
+    def permute(self, layout_old, layout_new):
+        self.psi = P.permute(self.psi, layout_old, layout_new)
+
+    def _apply(self, ops):
+        for o in ops:
+            self.psi = dense.apply(self.psi, o[-1], list(o[1:-1]), self.n)
+
+    apply_tile_cooperative = apply_tile_graphed = apply_tile_dev = _apply
+
+    def synchronize(self):
+        pass
+
+    def to_numpy(self):
+        return self.psi.astype(self.DTYPE)
+
+    def counters(self):
+        return {"kernel_launches": 0, "graph_launches": 0, "chunk_passes": 0}
+
+
+@pytest.mark.parametrize("mode", ["cooperative", "graphed", "sequential"])
+def test_host_driver_on_the_oracle(monkeypatch, mode):
+    """run_custom_backend's host side (backend.py:336-522: layouts, local
+    indices, fusion) with the oracle standing in for the library: the state
+    equals program-order simulation times exp(i global_phase); a circuit with
+    no gates returns exp(i phase)|0>."""
+    import types
+    from frontierlabproject_amd.backends.custom import backend as B
+    fake = types.SimpleNamespace(
+        Statevector=type("Statevector", (_OracleState,), {"DTYPE": np.complex128}),
+        StatevectorF32=type("StatevectorF32", (_OracleState,), {"DTYPE": np.complex64}))
+    monkeypatch.setattr(B, "qgpusim_cuda", fake)
+    n, phase = 7, -1.1
+    qc = W.random_circuit(n, 5, 77, measure_marker=True)
+    qc.global_phase = phase
+    pm, p = make_custom_pm(3)
+    tqc = pm.run(qc)
+    psi, m = B.run_custom_backend(tqc, p.property_set["flat_tiling_plan"], "statevector", tile_mode=mode)
+    ref = np.exp(1j * phase) * dense.simulate(C.to_dense_gates(C.random_circuit(n, 5, 77)), n)
+    assert psi.dtype == np.complex128 and np.max(np.abs(psi - ref)) < 1e-12
+    assert m["num_tiles"] == len(p.property_set["flat_tiling_plan"]) and m["tile_mode"] == mode
+    empty = qcirc.QuantumCircuit(n, phase)
+    empty.save_statevector()
+    pm, p = make_custom_pm(3)
+    te = pm.run(empty)
+    psi, _ = B.run_custom_backend(te, p.property_set["flat_tiling_plan"], "statevector", use_float32=True)
+    assert psi.dtype == np.complex64 and abs(psi[0] - np.exp(1j * phase)) < 1e-7 and np.count_nonzero(psi) == 1
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_fold_global_phase(seed):
+    """exp(i phase) folded into the first op: the list acts as the original
+    times the phase (FP64 oracle, global phase included); phase 0 or an empty
+    list changes nothing."""
+    from frontierlabproject_amd.backends.custom.backend import fold_global_phase
+    rng = np.random.default_rng(seed)
+    n = 5
+    ops = _tuples(C.random_circuit(n, 3, 40 + seed))
+    phase = float(rng.uniform(-3, 3))
+    folded = fold_global_phase(ops, phase, np.complex128)
+    assert len(folded) == len(ops) and folded[1:] == ops[1:]
+    psi0 = C.random_state(n, seed)
+    sim = lambda L: dense.simulate([(o[-1], list(o[1:-1])) for o in L], n, psi0)
+    assert np.max(np.abs(sim(folded) - np.exp(1j * phase) * sim(ops))) < 1e-12
+    assert fold_global_phase(ops, 0.0, np.complex128) is ops and fold_global_phase([], 1.0, np.complex64) == []
+
+
 def test_gate_cache_is_bounded_lru():
     """backend.py:16-85: at most MAX_CACHE_SIZE (512) entries (the reference
     evicts before inserting, so it holds one more); a hit refreshes an entry,
