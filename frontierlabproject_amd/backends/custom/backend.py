@@ -14,6 +14,10 @@ Differences from the reference, each deliberate:
 * fused matrices are multiplied in complex128 and then cast;
 * ``qc.global_phase`` is applied (the reference never reads it): folded into
   the matrix of the first op the circuit applies, so it costs no pass;
+* the state is returned in the circuit's qubit order also when the plan has
+  no final all-qubit tile (the reference then returns it in the last tile's
+  layout, passes.py:213-227): one lazy permute back to the identity layout,
+  folded into the read's permutation pass;
 * there is no CuPy fallback simulator: the HIP library is required.
 
 ``tile_mode``:
@@ -229,6 +233,12 @@ def run_custom_backend(qc, tile_plan: List[Tile], task: str, shots: int = None,
                 num_kernel_launches += len(tile_ops)
         t_apply += time.perf_counter() - t0
 
+    identity = list(range(num_qubits))
+    if layout != identity:  # no final all-qubit tile: back to the circuit's qubit order
+        t0 = time.perf_counter()
+        sv.permute(layout, identity)
+        t_perm += time.perf_counter() - t0
+        layout = identity
     t_sync0 = time.perf_counter()
     sv.synchronize()
     t_sync = time.perf_counter() - t_sync0
