@@ -14,6 +14,9 @@ Differences from the reference, each deliberate:
 * fused matrices are multiplied in complex128 and then cast;
 * ``qc.global_phase`` is applied (the reference never reads it): folded into
   the matrix of the first op the circuit applies, so it costs no pass;
+* ``use_cuda_graphs=False`` is honoured (the reference accepts and ignores
+  it, backend.py:336): ``tile_mode="graphed"`` then runs the same chunk passes
+  without capturing a graph, and the metrics report "cooperative";
 * the state is returned in the circuit's qubit order also when the plan has
   no final all-qubit tile (the reference then returns it in the last tile's
   layout, passes.py:213-227): one lazy permute back to the identity layout,
@@ -194,6 +197,8 @@ def run_custom_backend(qc, tile_plan: List[Tile], task: str, shots: int = None,
     """
     if tile_mode not in ("cooperative", "graphed", "sequential"):
         raise ValueError(f"unknown tile_mode {tile_mode!r}")
+    if tile_mode == "graphed" and not use_cuda_graphs:
+        tile_mode = "cooperative"  # the same chunk passes, no graph capture
     num_qubits = qc.num_qubits
     np_dtype = np.complex64 if use_float32 else np.complex128
     qubit_map = {q: qc.find_bit(q).index for q in qc.qubits}

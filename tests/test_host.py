@@ -389,6 +389,36 @@ def test_host_driver_on_the_oracle(monkeypatch, mode):
     assert np.max(np.abs(psi - ref)) < 1e-12
 
 
+def test_use_cuda_graphs_false_captures_no_graph(monkeypatch):
+    """use_cuda_graphs=False: "graphed" runs the tiles without graph capture
+    (the cooperative entry point) and reports so; True keeps graph replays."""
+    import types
+    from frontierlabproject_amd.backends.custom import backend as B
+    calls = []
+
+    class Rec(_OracleState):
+        DTYPE = np.complex128
+
+        def apply_tile_graphed(self, ops):
+            calls.append("graphed")
+            self._apply(ops)
+
+        def apply_tile_cooperative(self, ops):
+            calls.append("cooperative")
+            self._apply(ops)
+
+    monkeypatch.setattr(B, "qgpusim_cuda", types.SimpleNamespace(Statevector=Rec, StatevectorF32=Rec))
+    qc = W.random_circuit(6, 3, 5, measure_marker=True)
+    pm, p = make_custom_pm(3)
+    tqc = pm.run(qc)
+    plan = p.property_set["flat_tiling_plan"]
+    _, m = B.run_custom_backend(tqc, plan, "statevector", tile_mode="graphed", use_cuda_graphs=False)
+    assert set(calls) == {"cooperative"} and m["tile_mode"] == "cooperative" and m["num_graph_replays"] == 0
+    calls.clear()
+    _, m = B.run_custom_backend(tqc, plan, "statevector", tile_mode="graphed")
+    assert set(calls) == {"graphed"} and m["tile_mode"] == "graphed" and m["num_graph_replays"] == len(calls)
+
+
 @pytest.mark.parametrize("seed", range(4))
 def test_fold_global_phase(seed):
     """exp(i phase) folded into the first op: the list acts as the original
