@@ -10,8 +10,7 @@ twoq_count, cx_count) and the metrics run_custom_backend returned
 ("aer_rows": the Aer baseline rows, transpiled at optimisation level 0, so
 before any gate merging); plus each kept row verbatim ("csv_row", every
 column as written but the four naming the machine and time) and the file's
-header, which pin the schema and the
-number formats of a result row.
+header, which pin the schema and the number formats of a result row.
 
 Run once where the reference is present:
     python tests/golden/make_result_csv_golden.py /root/reference/results/result.csv
@@ -23,8 +22,8 @@ import sys
 
 SHAPE_FIELDS = ("nqubits", "depth", "nqubits_t", "depth_t", "total_gates", "twoq_count", "cx_count", "cz_count",
                 "swap_count", "rz_count", "rx_count")
-INT_FIELDS = SHAPE_FIELDS + ("nL", "num_tiles", "num_gates_custom", "num_fused_ops", "num_reorders", "num_kernel_launches",
-              "num_graph_replays", "avoided_launches")
+INT_FIELDS = SHAPE_FIELDS + ("nL", "num_tiles", "num_gates_custom", "num_fused_ops", "num_reorders",
+                             "num_kernel_launches", "num_graph_replays", "avoided_launches")
 FLOAT_FIELDS = ("fusion_ratio", "t_launch_overhead_saved_us")
 STR_FIELDS = ("circuit", "task", "tile_mode", "backend")
 ENV_FIELDS = ("timestamp", "host", "os", "python")  # where and when the run was made: not kept
