@@ -749,7 +749,8 @@ def test_chunk_plan_keeps_probe_order(hip):
     assert sorted(order) == list(range(n))
 
 
-@pytest.mark.parametrize("dtype,K,cmin", [(np.complex64, 12, 4), (np.complex128, 11, 3), (np.complex64, 7, 2)])
+@pytest.mark.parametrize("dtype,K,cmin", [(np.complex64, 12, 4), (np.complex64, 13, 4), (np.complex128, 11, 3),
+                                         (np.complex128, 12, 3), (np.complex64, 7, 2)])
 def test_specialised_pass_kernels_compile(dtype, K, cmin):
     """Every pass kernel generated for a plan with 1q kinds, pair slots and dense
     steps compiles for gfx950 (hiprtc, no device needed)."""
@@ -842,14 +843,15 @@ def test_batch_spec_and_generator_are_reproducible():
     assert [ci.operation.name for ci in qc.data] == [g[0] for g in stream]
 
 
-@pytest.mark.parametrize("dtype", [np.complex64, np.complex128])
+@pytest.mark.parametrize("dtype,K,cmin", [(np.complex64, 12, 4), (np.complex64, 13, 4), (np.complex128, 11, 3),
+                                         (np.complex128, 12, 3)])
 @pytest.mark.parametrize("seed", [1, 2, 3])
-def test_folded_layout_change_kernels_compile(seed, dtype):
+def test_folded_layout_change_kernels_compile(seed, dtype, K, cmin):
     """First passes that fold a pending layout change (read the old layout,
-    write the new one) compile for gfx950, with and without gates."""
+    write the new one) compile for gfx950, with and without gates, at the
+    default chunk geometries (13/4 complex64, 12/3 complex128) and one below."""
     from frontierlabproject_amd.backends.custom.native import qgpusim_hip as hip
     n = 16
-    K, cmin = (12, 4) if dtype == np.complex64 else (11, 3)
     rng = np.random.default_rng(seed)
     in_map = rng.permutation(n).astype(np.int32)
     ops = [("1q", int(q), G.h()) for q in rng.integers(0, n, 6)] + [("2q", 0, 5, G.cx())]
