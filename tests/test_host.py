@@ -205,6 +205,23 @@ def test_lqr_pass_matches_oracle(seed):
         [idx_p[id(nd.op)] for nd in lqr_pass.property_set["flat_tiling_schedule"]]
 
 
+@pytest.mark.parametrize("name,n,depth,nL,seed", [("random", 33, 30, 18, 1002), ("random", 20, 20, 10, 1000),
+                                                    ("qft", 30, 0, 16, 0)])
+def test_lqr_pass_matches_oracle_at_benchmark_sizes(name, n, depth, nL, seed):
+    """The product pass and the oracle's literal restatement give the same
+    tiles (qubit sets, layouts, gates in order) on the benchmark workloads at
+    their full sizes."""
+    ops = (C.random_circuit(n, depth, seed) if name == "random" else C.qft_circuit(n)) + [C.marker(n)]
+    oplan, _, _ = lqr.lqr(ops, n, nL)
+    qc = _to_product(ops, n)
+    pm, lqr_pass = make_custom_pm(nL)
+    pm.run(qc)
+    idx_o = {id(o): i for i, o in enumerate(ops)}
+    idx_p = {id(ci.operation): i for i, ci in enumerate(qc.data)}
+    assert _plan_signature(oplan, lambda g: idx_o[id(g)]) == \
+        _plan_signature(lqr_pass.property_set["flat_tiling_plan"], lambda nd: idx_p[id(nd.op)])
+
+
 @pytest.mark.parametrize("row", result_csv_rows(), ids=lambda r: f"csv_line{r['csv_line']}")
 def test_lqr_pass_reference_known_answers(row):
     """The published rows with general 1q gates (u) in place of the merged
