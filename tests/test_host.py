@@ -795,6 +795,28 @@ def test_bench_refuses_more_than_one_gpu():
     assert "one GPU" in (r.stdout + r.stderr)
 
 
+def test_random33_plans_keep_per_qubit_order(hip):
+    """Every tile of the 33-qubit depth-30 program, planned at the default
+    geometries of both precisions: the execution order is a permutation of
+    the ops that keeps each qubit's program order (what makes it equal to
+    program order; checked on states up to 10 qubits elsewhere)."""
+    import bench
+    for dtype, K, cmin in ((np.complex64, 13, 4), (np.complex128, 12, 3)):
+        _, _, prog, _, _, _ = bench.build_program(hip, 33, 30, 18, 1002, dtype)
+        for it in prog:
+            if it[0] != "tile":
+                continue
+            _, arr, nops, arena = it
+            _, order = hip.plan_describe(33, arr, nops, arena, K, cmin)
+            assert sorted(order) == list(range(nops))
+            last = {}
+            for i in order:
+                qs = [arr[i].q0] + ([arr[i].q1] if arr[i].kind == hip.OP_2Q else [])
+                for q in qs:
+                    assert last.get(q, -1) < i
+                    last[q] = i
+
+
 def test_random33_tiles_plan_in_two_passes(hip):
     """Every tile of the 33-qubit depth-30 program (18 qubits each, so two
     13-bit chunks at least) plans in exactly two passes."""
