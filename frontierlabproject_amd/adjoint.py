@@ -120,6 +120,11 @@ class AdjointSimulator:
         self.sv = hip.StatevectorF32(num_qubits) if use_float32 else hip.Statevector(num_qubits)
         self.nparams = 1 + max([k for _, _, k, _ in self.spec if k is not None], default=-1)
         self._probe_param = [k for _, _, k, _ in self.spec if k is not None]
+        # Bumped whenever the state changes: a caller that saw generation g
+        # after prepare() knows the state still holds that psi(theta) while
+        # the generation is g.
+        self.generation = 0
+        self._theta = None
 
     def _forward_ops(self, theta):
         return forward_ops(self.spec, theta, self.dtype)
@@ -135,6 +140,7 @@ class AdjointSimulator:
         else:
             self.sv.from_numpy(psi0)
         buf = self._forward_ops(theta)
+        self.generation += 1
         self.sv.apply_packed(hip.COOPERATIVE, buf.arr, buf.n, buf.arena())
         self._theta = theta
 
@@ -149,8 +155,12 @@ class AdjointSimulator:
     def gradient_from_state(self):
         """Adjoint sweep from the state left by prepare()/forward().
         Returns (E, grad); the state is un-computed back to the input."""
+        if self._theta is None:
+            raise RuntimeError("gradient_from_state: no prepared state (call prepare or forward first)")
         buf, nprobe = self._reverse_ops(self._theta)
+        self.generation += 1  # the sweep un-computes the state
         probes, e = self.sv.adjoint(self.observable, buf.arr, buf.n, buf.arena(), nprobe)
+        self._theta = None
         grad = np.zeros(self.nparams, dtype=np.float64)
         np.add.at(grad, np.asarray(self._probe_param, dtype=np.int64), probes)
         return e, grad
@@ -168,13 +178,23 @@ def _make_function():
 
         @staticmethod
         def forward(ctx, theta, sim):
-            e = sim.forward(theta.detach().cpu().double().numpy())
+            th = theta.detach().cpu().double().numpy().copy()
+            e = sim.forward(th)
             ctx.sim = sim
+            ctx.theta = th
+            ctx.generation = sim.generation
             return torch.tensor(e, dtype=torch.float64, device=theta.device)
 
         @staticmethod
         def backward(ctx, grad_out):
-            _, g = ctx.sim.gradient_from_state()
+            sim = ctx.sim
+            if sim.generation == ctx.generation:
+                # the state is still this forward's psi(theta): sweep it
+                _, g = sim.gradient_from_state()
+            else:
+                # another forward (or a sweep) ran on the simulator since:
+                # prepare this forward's state again
+                _, g = sim.value_and_grad(ctx.theta)
             gt = torch.as_tensor(g, dtype=torch.float64, device=grad_out.device)
             return grad_out * gt, None
 

@@ -47,20 +47,27 @@ KERNEL_LAUNCH_OVERHEAD_US = 10.0
 
 
 def _gate_key(op, dtype):
-    try:
-        params = tuple(float(p) for p in getattr(op, "params", []))
-    except TypeError:
-        params = None
-    if params is None and getattr(op, "name", "") == "unitary":
-        return None  # matrices of parameterless unitaries are not cached by name
-    return (op.name, params, np.dtype(dtype).str)
+    """Cache key (name, float params, dtype), or None when the matrix must not
+    be cached: a unitary given by its matrix, or any parameter that is not a
+    real scalar (an array, a string, a symbolic expression), since two such
+    gates of one name can hold different matrices."""
+    name = getattr(op, "name", "")
+    if name == "unitary":
+        return None
+    params = []
+    for p in getattr(op, "params", []):
+        if isinstance(p, (str, bytes)) or np.ndim(p) != 0:
+            return None
+        try:
+            params.append(float(p))
+        except (TypeError, ValueError):
+            return None
+    return (name, tuple(params), np.dtype(dtype).str)
 
 
 def get_gate_matrix_host(op, dtype=np.complex128):
     """Matrix of ``op`` as a C-contiguous array of ``dtype`` (LRU-cached)."""
     key = _gate_key(op, dtype)
-    if key is not None and key[0] == "unitary":
-        key = None
     if key is not None:
         hit = GATE_CACHE.get(key)
         if hit is not None:
@@ -286,6 +293,8 @@ def run_custom_backend(qc, tile_plan: List[Tile], task: str, shots: int = None,
         "device_kernel_launches": counters["kernel_launches"],
         "device_graph_launches": counters["graph_launches"],
         "device_chunk_passes": counters["chunk_passes"],
+        "device_folded_permutations": counters["folded_permutations"],
+        "device_permutation_passes": counters["permutations"],
     }
     if counts is not None:
         metrics["counts"] = counts

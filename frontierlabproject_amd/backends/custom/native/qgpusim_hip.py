@@ -312,8 +312,9 @@ class _StatevectorBase:
         return out.value
 
     def overlap(self, other):
-        if type(other) is not type(self):
-            raise RuntimeError("overlap: states differ in size or precision")
+        """<self|other> in FP64; the two states may differ in precision."""
+        if not isinstance(other, _StatevectorBase):
+            raise RuntimeError("overlap: other is not a statevector")
         re, im = _D(), _D()
         _check(lib.qsim_overlap(self._h, other._h, ctypes.byref(re), ctypes.byref(im)))
         return complex(re.value, im.value)
@@ -401,9 +402,15 @@ class StatevectorF32(_StatevectorBase):
     _CODE = C64
 
 
-def _raw_target(psi):
+def _raw_target(psi, num_qubits):
     """(device pointer, dtype code, stream) of a torch CUDA tensor or an object
-    exposing __cuda_array_interface__."""
+    exposing __cuda_array_interface__, after checking that it is a dense
+    vector of exactly 2^num_qubits complex elements (the kernel touches that
+    many amplitudes, so any other size would run past the caller's buffer)."""
+    n = int(num_qubits)
+    if n < 1 or n > 40:
+        raise RuntimeError("num_qubits must be in [1, 40]")
+    dim = 1 << n
     if hasattr(psi, "data_ptr") and hasattr(psi, "is_cuda"):
         import torch
         if not psi.is_cuda:
@@ -416,6 +423,8 @@ def _raw_target(psi):
             raise RuntimeError("psi must be complex64 or complex128")
         if not psi.is_contiguous():
             raise RuntimeError("psi must be contiguous")
+        if psi.numel() != dim:
+            raise RuntimeError("psi must have 2^num_qubits elements")
         return psi.data_ptr(), code, torch.cuda.current_stream(psi.device).cuda_stream
     iface = getattr(psi, "__cuda_array_interface__", None)
     if iface is None:
@@ -427,19 +436,33 @@ def _raw_target(psi):
     code = C128 if typestr == np.complex128 else C64 if typestr == np.complex64 else None
     if code is None:
         raise RuntimeError("psi must be complex64 or complex128")
+    shape = tuple(int(x) for x in iface.get("shape", ()))
+    count = 1
+    for x in shape:
+        count *= x
+    if count != dim:
+        raise RuntimeError("psi must have 2^num_qubits elements")
+    strides = iface.get("strides")
+    if strides is not None:
+        # C-contiguous strides of `shape` (extent-1 axes may carry any stride)
+        expect = typestr.itemsize
+        for x, st in zip(reversed(shape), reversed(tuple(strides))):
+            if x > 1 and int(st) != expect:
+                raise RuntimeError("psi must be contiguous")
+            expect *= x
     return ptr, code, None
 
 
 def apply_1q_gate(psi, U, target_qubit, num_qubits):
     """Module function apply_1q_gate (bindings.cpp:931-962)."""
-    ptr, code, stream = _raw_target(psi)
+    ptr, code, stream = _raw_target(psi, num_qubits)
     m = _matrix(U, 1, np.complex128 if code == C128 else np.complex64)
     _check(lib.qsim_apply_1q_raw(ptr, code, int(num_qubits), int(target_qubit), m.ctypes.data, stream))
 
 
 def apply_2q_gate(psi, U, q0, q1, num_qubits):
     """Module function apply_2q_gate (bindings.cpp:965-1001)."""
-    ptr, code, stream = _raw_target(psi)
+    ptr, code, stream = _raw_target(psi, num_qubits)
     m = _matrix(U, 2, np.complex128 if code == C128 else np.complex64)
     if int(q0) == int(q1):
         raise RuntimeError("q0 and q1 must be different")

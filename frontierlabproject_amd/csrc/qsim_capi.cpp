@@ -609,13 +609,26 @@ int run_graphed(qsim_state *st, const qsim_op *ops, int nops, const void *mats, 
     HIP_TRY(hipMalloc(&e.desc, img.bytes.size()));
     HIP_TRY(hipMemcpyAsync(e.desc, img.bytes.data(), img.bytes.size(), hipMemcpyHostToDevice, st->stream));
     HIP_TRY(hipStreamSynchronize(st->stream));
-    HIP_TRY(hipStreamBeginCapture(st->stream, hipStreamCaptureModeThreadLocal));
+    // Capturing a folded first pass swaps psi and tmp (enqueue_passes), but no
+    // pass runs until the launch below: every failure return before it puts
+    // the buffers back, so the state keeps describing the data it holds.
+    void *const psi0 = st->psi, *const tmp0 = st->tmp;
+    auto undo = [&]() {
+        st->psi = psi0;
+        st->tmp = tmp0;
+    };
+    hipError_t eb = hipStreamBeginCapture(st->stream, hipStreamCaptureModeThreadLocal);
+    if (eb != hipSuccess) {
+        hipFree(e.desc);
+        return fail(std::string("HIP error: ") + hipGetErrorString(eb));
+    }
     st->capturing = true;
     int rc = enqueue_passes<R>(st, plan, img, static_cast<char *>(e.desc), nullptr, nullptr, &fns);
     st->capturing = false;
     hipGraph_t g = nullptr;
     hipError_t ec = hipStreamEndCapture(st->stream, &g);
     if (rc || ec != hipSuccess) {
+        undo();
         if (g) hipGraphDestroy(g);
         hipFree(e.desc);
         if (rc) return rc;
@@ -624,13 +637,23 @@ int run_graphed(qsim_state *st, const qsim_op *ops, int nops, const void *mats, 
     e.graph = g;
     hipError_t ei = hipGraphInstantiate(&e.exec, g, nullptr, nullptr, 0);
     if (ei != hipSuccess) {
+        undo();
         hipGraphDestroy(g);
         hipFree(e.desc);
         return fail(std::string("HIP error: ") + hipGetErrorString(ei));
     }
     st->counters[2]++;
     st->graphs.push_front(std::move(e));
-    HIP_TRY(hipGraphLaunch(st->graphs.front().exec, st->stream));
+    hipError_t el = hipGraphLaunch(st->graphs.front().exec, st->stream);
+    if (el != hipSuccess) {
+        undo();
+        GraphEntry &bad = st->graphs.front();
+        hipGraphExecDestroy(bad.exec);
+        hipGraphDestroy(bad.graph);
+        hipFree(bad.desc);
+        st->graphs.pop_front();
+        return fail(std::string("HIP error: ") + hipGetErrorString(el));
+    }
     st->counters[1]++;
     set_layout(st, after);
     st->zero_pending = false;
@@ -718,13 +741,13 @@ int permute_map_impl(int n, const int32_t *lo, const int32_t *ln, int32_t *map)
     return 0;
 }
 
-template <typename R>
+template <typename RA, typename RB = RA>
 int dot_t(qsim_state *st, const void *a, const void *b, double *re, double *im)
 {
     if (ensure_small(st, 2 * (size_t)max_dot_blocks() + 8)) return -1;
     int nb = 0;
-    HIP_TRY(launch_dot<R>(static_cast<const cx<R> *>(a), static_cast<const cx<R> *>(b), st->dim, st->d_small, &nb,
-                          st->stream));
+    HIP_TRY((launch_dot<RA, RB>(static_cast<const cx<RA> *>(a), static_cast<const cx<RB> *>(b), st->dim,
+                                st->d_small, &nb, st->stream)));
     double *out = st->d_small + 2 * (size_t)max_dot_blocks();
     HIP_TRY(launch_sum_partials(st->d_small, nb, 2, out, st->stream));
     double h[2];
@@ -1115,11 +1138,15 @@ int qsim_norm2(qsim_state *st, double *out)
 int qsim_overlap(qsim_state *a, qsim_state *b, double *re, double *im)
 {
     if (!a || !b || !re || !im) return fail("null argument");
-    if (a->n != b->n || a->dtype != b->dtype) return fail("overlap: states differ in size or precision");
+    if (a->n != b->n) return fail("overlap: states differ in size");
+    if (a->device != b->device) return fail("overlap: states live on different devices");
     if (flush_perm(a) || flush_perm(b)) return -1;
     HIP_TRY(hipStreamSynchronize(b->stream));
-    if (a->dtype == QSIM_C64) return dot_t<float>(a, a->psi, b->psi, re, im);
-    return dot_t<double>(a, a->psi, b->psi, re, im);
+    if (a->dtype == QSIM_C64)
+        return b->dtype == QSIM_C64 ? dot_t<float>(a, a->psi, b->psi, re, im)
+                                    : dot_t<float, double>(a, a->psi, b->psi, re, im);
+    return b->dtype == QSIM_C64 ? dot_t<double, float>(a, a->psi, b->psi, re, im)
+                                : dot_t<double>(a, a->psi, b->psi, re, im);
 }
 
 int qsim_scale(qsim_state *st, double factor)

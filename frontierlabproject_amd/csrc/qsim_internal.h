@@ -62,8 +62,8 @@ template <typename R>
 hipError_t launch_pauli_chunk(const cx<R> *psi, cx<R> *lam, int n, const PassDev &host_pass,
                               const PassDev *d_pass, const PauliDev *d_terms, int nterms,
                               int mode, double *d_partial, hipStream_t s);
-template <typename R>
-hipError_t launch_dot(const cx<R> *a, const cx<R> *b, uint64_t dim, double *d_partial,
+template <typename RA, typename RB>
+hipError_t launch_dot(const cx<RA> *a, const cx<RB> *b, uint64_t dim, double *d_partial,
                       int *nblocks_out, hipStream_t s);
 hipError_t launch_sum_partials(const double *partial, int nblocks, int width, double *out,
                                hipStream_t s);

@@ -495,16 +495,18 @@ hipError_t launch_probe_reduce(const double *partial, int nblocks, int nprobe, c
 constexpr int kDotBlocks = 2048;
 int max_dot_blocks() { return kDotBlocks; }
 
-// partial[b*2 + {0,1}] = sum conj(a_i) b_i over the block's grid-stride share
-template <typename R>
-__global__ __launch_bounds__(kTPB) void k_dot(const cx<R> *__restrict__ a, const cx<R> *__restrict__ b, uint64_t dim,
+// partial[b*2 + {0,1}] = sum conj(a_i) b_i over the block's grid-stride share;
+// the two vectors may differ in precision (every product is taken in FP64)
+template <typename RA, typename RB>
+__global__ __launch_bounds__(kTPB) void k_dot(const cx<RA> *__restrict__ a, const cx<RB> *__restrict__ b, uint64_t dim,
                                               double *__restrict__ partial)
 {
     __shared__ double sh[kWaves];
     NSum re, im;
     const uint64_t stride = (uint64_t)gridDim.x * kTPB;
     for (uint64_t i = (uint64_t)blockIdx.x * kTPB + threadIdx.x; i < dim; i += stride) {
-        const cx<R> x = a[i], y = b[i];
+        const cx<RA> x = a[i];
+        const cx<RB> y = b[i];
         const double xr = x.re, xi = x.im, yr = y.re, yi = y.im;
         re.add(xr * yr + xi * yi);
         im.add(xr * yi - xi * yr);
@@ -515,16 +517,26 @@ __global__ __launch_bounds__(kTPB) void k_dot(const cx<R> *__restrict__ a, const
     if (threadIdx.x == 0) partial[2 * blockIdx.x + 1] = t;
 }
 
-template <typename R>
-hipError_t launch_dot(const cx<R> *a, const cx<R> *b, uint64_t dim, double *d_partial, int *nblocks_out, hipStream_t s)
+template <typename RA, typename RB>
+hipError_t launch_dot(const cx<RA> *a, const cx<RB> *b, uint64_t dim, double *d_partial, int *nblocks_out,
+                      hipStream_t s)
 {
     uint64_t nb = (dim + kTPB * 8 - 1) / (kTPB * 8);
     if (nb > kDotBlocks) nb = kDotBlocks;
     if (nb < 1) nb = 1;
     *nblocks_out = (int)nb;
-    hipLaunchKernelGGL(k_dot<R>, dim3((unsigned)nb), dim3(kTPB), 0, s, a, b, dim, d_partial);
+    hipLaunchKernelGGL((k_dot<RA, RB>), dim3((unsigned)nb), dim3(kTPB), 0, s, a, b, dim, d_partial);
     return hipGetLastError();
 }
+
+template hipError_t launch_dot<float, float>(const cx<float> *, const cx<float> *, uint64_t, double *, int *,
+                                             hipStream_t);
+template hipError_t launch_dot<double, double>(const cx<double> *, const cx<double> *, uint64_t, double *, int *,
+                                               hipStream_t);
+template hipError_t launch_dot<float, double>(const cx<float> *, const cx<double> *, uint64_t, double *, int *,
+                                              hipStream_t);
+template hipError_t launch_dot<double, float>(const cx<double> *, const cx<float> *, uint64_t, double *, int *,
+                                              hipStream_t);
 
 // out[w] = sum_b partial[b * width + w], one block per w, fixed order.
 __global__ __launch_bounds__(kTPB) void k_sum_partials(const double *__restrict__ partial, int nblocks, int width,
@@ -682,7 +694,6 @@ hipError_t prepare_kernels()
     template hipError_t launch_pauli_chunk<R>(const cx<R> *, cx<R> *, int, const PassDev &,                 \
                                               const PassDev *, const PauliDev *, int, int, double *,        \
                                               hipStream_t);                                                 \
-    template hipError_t launch_dot<R>(const cx<R> *, const cx<R> *, uint64_t, double *, int *, hipStream_t); \
     template hipError_t launch_scale<R>(cx<R> *, uint64_t, double, hipStream_t);
 INST(float)
 INST(double)

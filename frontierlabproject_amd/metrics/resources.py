@@ -1,10 +1,15 @@
 """Circuit statistics and resource snapshots (qgpusim/metrics/resources.py).
 
 Same keys and string formats as the reference, so result rows line up.  The
-reference reads GPU memory and utilisation through NVML; here device memory
-in use comes from hipMemGetInfo (device-wide, in MiB) and utilisation is not
-sampled (left empty, as the reference leaves it when NVML is missing).
+reference reads this process's VRAM and the device's utilisation through NVML
+(resources.py:39-86).  On ROCm the same figures come from the kernel driver:
+the process's VRAM from the KFD's per-process counters
+(/sys/class/kfd/kfd/proc/<pid>/vram_<gpu id>, bytes, summed over GPUs) and
+the utilisation from the first GPU's gpu_busy_percent in sysfs.  Without the
+KFD counters the device-wide memory in use (hipMemGetInfo) stands in; without
+the busy counter gpu_util stays empty, as the reference leaves it without NVML.
 """
+import glob
 import os
 from typing import Dict
 
@@ -28,8 +33,31 @@ def _tqc_stats(tqc) -> Dict[str, int]:
     }
 
 
+def _process_vram_bytes():
+    """VRAM held by this process on every GPU (KFD counters), or None."""
+    files = glob.glob(f"/sys/class/kfd/kfd/proc/{os.getpid()}/vram_*")
+    if not files:
+        return None
+    total = 0
+    for path in files:
+        with open(path) as f:
+            total += int(f.read().strip() or 0)
+    return total
+
+
+def _gpu_busy_percent():
+    """Instantaneous utilisation of the first GPU (sysfs), or None."""
+    for path in sorted(glob.glob("/sys/class/drm/card*/device/gpu_busy_percent")):
+        try:
+            with open(path) as f:
+                return int(f.read().strip())
+        except (OSError, ValueError):
+            continue
+    return None
+
+
 def _mem_snapshot() -> Dict[str, str]:
-    """Process RSS and device memory in use, in MiB (resources.py:39-86)."""
+    """Process RSS, this process's VRAM (MiB) and GPU utilisation (%)."""
     out = {"rss_mb": "", "gpu_mem_mb": "", "gpu_util": ""}
     try:
         import psutil
@@ -37,34 +65,50 @@ def _mem_snapshot() -> Dict[str, str]:
     except Exception:
         pass
     try:
-        from ..backends.custom.native import qgpusim_hip as hip
-        free, total = hip.device_memory()
-        out["gpu_mem_mb"] = f"{(total - free) / (1024 ** 2):.2f}"
+        used = _process_vram_bytes()
+        if used is None:
+            from ..backends.custom.native import qgpusim_hip as hip
+            free, total = hip.device_memory()
+            used = total - free
+        out["gpu_mem_mb"] = f"{used / (1024 ** 2):.2f}"
     except Exception:
         pass
+    busy = _gpu_busy_percent()
+    if busy is not None:
+        out["gpu_util"] = str(busy)
     return out
 
 
-def _to_float(x):
+# CSV form of each usage field: memory in MiB with two decimals, utilisation
+# as a whole percentage (fraction dropped).
+_USAGE_FORMAT = {"rss_mb": lambda v: f"{v:.2f}", "gpu_mem_mb": lambda v: f"{v:.2f}",
+                 "gpu_util": lambda v: str(int(v))}
+
+
+def _number(text):
+    """The field's value as a float, or None when it holds no number."""
     try:
-        return float(x)
-    except Exception:
+        return float(text)
+    except (TypeError, ValueError):
         return None
 
 
 def _max_usage(a, b):
-    """Field-wise maximum of two snapshots, kept as CSV-style strings
-    (resources.py:96-127)."""
-    out = dict(a) if a else {"rss_mb": "", "gpu_mem_mb": "", "gpu_util": ""}
-    for k in ("rss_mb", "gpu_mem_mb"):
-        av, bv = _to_float(out.get(k, "")), _to_float(b.get(k, ""))
-        if av is None:
-            out[k] = b.get(k, out.get(k, ""))
-        elif bv is not None:
-            out[k] = f"{max(av, bv):.2f}"
-    av, bv = _to_float(out.get("gpu_util", "")), _to_float(b.get("gpu_util", ""))
-    if av is None:
-        out["gpu_util"] = b.get("gpu_util", out.get("gpu_util", ""))
-    elif bv is not None:
-        out["gpu_util"] = str(int(max(av, bv)))
-    return out
+    """Peak of two usage snapshots, field by field.
+
+    Rule: when both fields hold numbers the result is the larger, written in
+    the field's CSV format; when only one does, that side's text is kept as it
+    is; a field that no number fills on the first side takes the second
+    side's text (or stays as it was when the second side lacks the field).
+    A missing first snapshot counts as all fields empty."""
+    merged = {k: "" for k in _USAGE_FORMAT}
+    if a:
+        merged.update(a)
+    for key, fmt in _USAGE_FORMAT.items():
+        mine, theirs = merged.get(key, ""), b.get(key, merged.get(key, ""))
+        x, y = _number(mine), _number(theirs)
+        if x is None:
+            merged[key] = theirs
+        elif y is not None:
+            merged[key] = fmt(max(x, y))
+    return merged

@@ -12,6 +12,7 @@
   (xmask, zmask, coef) terms.
 """
 import math
+import os
 
 import numpy as np
 
@@ -111,7 +112,32 @@ def ring_observable(n):
     return terms
 
 
-def random_state(n, seed):
-    rng = np.random.Generator(np.random.PCG64(seed))
-    psi = rng.standard_normal(1 << n) + 1j * rng.standard_normal(1 << n)
-    return psi / np.linalg.norm(psi)
+RANDOM_STATE_STREAMS = 16
+
+
+def random_state(n, seed, workers=None):
+    """Seeded state with i.i.d. complex Gaussian amplitudes (real and
+    imaginary parts standard normal), normalised in FP64, as complex128.
+
+    The 2^(n+1) normals are drawn by RANDOM_STATE_STREAMS independent PCG64
+    streams spawned from SeedSequence(seed), stream k filling the k-th equal
+    slice of the interleaved (re, im) vector, so the state depends on the
+    seed only, not on how many threads (``workers``, default all host cores,
+    at most 16) draw it; at 30 qubits this takes seconds instead of a
+    minute."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    psi = np.empty(1 << n, dtype=np.complex128)
+    flat = psi.view(np.float64)
+    nstreams = min(RANDOM_STATE_STREAMS, flat.size)
+    seqs = np.random.SeedSequence(seed).spawn(nstreams)
+    step = flat.size // nstreams
+
+    def draw(k):
+        np.random.Generator(np.random.PCG64(seqs[k])).standard_normal(out=flat[k * step:(k + 1) * step])
+
+    workers = workers or min(16, os.cpu_count() or 1)
+    with ThreadPoolExecutor(max_workers=max(1, workers)) as ex:
+        list(ex.map(draw, range(nstreams)))
+    psi /= math.sqrt(np.vdot(psi, psi).real)
+    return psi

@@ -141,7 +141,9 @@ int qsim_apply_2q_raw(void *psi, int dtype, int num_qubits, int q0, int q1, cons
 /* <psi|psi> in FP64 (Neumaier-compensated, fixed order).  Synchronises. */
 int qsim_norm2(qsim_state *st, double *out);
 
-/* <a|b> in FP64.  Both states need the same size and precision.  Synchronises. */
+/* <a|b> in FP64.  Both states need the same size; they may differ in
+ * precision (a complex64 result judged against its complex128 run).
+ * Synchronises. */
 int qsim_overlap(qsim_state *a, qsim_state *b, double *re, double *im);
 
 /* Scale every amplitude by `factor` (periodic renormalisation). */

@@ -101,10 +101,16 @@ def ansatz_params(n, layers, seed=1003):
 
 
 def random_state(n, seed):
-    """i.i.d. complex Gaussian amplitudes, normalised in FP64."""
-    rng = np.random.Generator(np.random.PCG64(seed))
-    psi = rng.standard_normal(1 << n) + 1j * rng.standard_normal(1 << n)
-    return psi / np.linalg.norm(psi)
+    """i.i.d. complex Gaussian amplitudes, normalised in FP64: the interleaved
+    (re, im) vector of 2^(n+1) standard normals, its k-th of 16 equal slices
+    drawn by the k-th PCG64 stream spawned from SeedSequence(seed)."""
+    flat = np.empty(2 << n, dtype=np.float64)
+    streams = min(16, flat.size)
+    step = flat.size // streams
+    for k, ss in enumerate(np.random.SeedSequence(seed).spawn(streams)):
+        flat[k * step:(k + 1) * step] = np.random.Generator(np.random.PCG64(ss)).standard_normal(step)
+    psi = flat.view(np.complex128)
+    return psi / np.sqrt(np.vdot(psi, psi).real)
 
 
 def marker(n):

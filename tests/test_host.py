@@ -365,7 +365,8 @@ class _OracleState:
         return self.psi.astype(self.DTYPE)
 
     def counters(self):
-        return {"kernel_launches": 0, "graph_launches": 0, "chunk_passes": 0}
+        return {"kernel_launches": 0, "graph_launches": 0, "chunk_passes": 0, "folded_permutations": 0,
+                "permutations": 0}
 
     def sample(self, shots, seed=0):
         p = np.abs(self.psi) ** 2

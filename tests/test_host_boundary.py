@@ -1,0 +1,165 @@
+"""Host-side behaviour at the boundary (no GPU): the autograd wrapper's
+bookkeeping when one simulator serves several forwards, the refusals of the
+module gate functions before any device call, the gate-matrix cache key, and
+the usage-snapshot merge rule."""
+import numpy as np
+import pytest
+
+
+class _FakeSim:
+    """Stands in for AdjointSimulator with E(theta) = sum_k c_k sin(theta_k),
+    keeping the real one's state semantics: forward() prepares psi(theta) and
+    bumps the generation; gradient_from_state() sweeps the prepared state (and
+    un-computes it); value_and_grad() prepares, then sweeps."""
+
+    def __init__(self, coef):
+        self.coef = np.asarray(coef, dtype=np.float64)
+        self.generation = 0
+        self._theta = None
+        self.sweeps = 0
+
+    def forward(self, theta):
+        self._theta = np.asarray(theta, dtype=np.float64).copy()
+        self.generation += 1
+        return float(np.sum(self.coef * np.sin(self._theta)))
+
+    def gradient_from_state(self):
+        if self._theta is None:
+            raise RuntimeError("no prepared state")
+        g = self.coef * np.cos(self._theta)
+        e = float(np.sum(self.coef * np.sin(self._theta)))
+        self._theta = None
+        self.generation += 1
+        self.sweeps += 1
+        return e, g
+
+    def value_and_grad(self, theta):
+        self.forward(theta)
+        return self.gradient_from_state()
+
+
+def test_autograd_two_forwards_one_backward():
+    """E(a) + E(b) on one simulator: each input gets its own gradient (the
+    second forward replaced the state the first one left)."""
+    import torch
+    from frontierlabproject_amd.adjoint import adjoint_expectation
+    sim = _FakeSim([1.0, 2.0, -0.5])
+    a = torch.tensor([0.1, 0.7, -1.2], dtype=torch.float64, requires_grad=True)
+    b = torch.tensor([1.3, -0.4, 2.2], dtype=torch.float64, requires_grad=True)
+    (adjoint_expectation(a, sim) + 3.0 * adjoint_expectation(b, sim)).backward()
+    assert np.allclose(a.grad.numpy(), sim.coef * np.cos(a.detach().numpy()), atol=1e-15)
+    assert np.allclose(b.grad.numpy(), 3.0 * sim.coef * np.cos(b.detach().numpy()), atol=1e-15)
+
+
+def test_autograd_retain_graph_double_backward():
+    """A second backward over a retained graph re-prepares psi(theta) (the
+    first sweep un-computed it) and gives the same gradient again."""
+    import torch
+    from frontierlabproject_amd.adjoint import adjoint_expectation
+    sim = _FakeSim([0.3, -1.1])
+    th = torch.tensor([0.9, 2.5], dtype=torch.float64, requires_grad=True)
+    e = adjoint_expectation(th, sim)
+    e.backward(retain_graph=True)
+    g1 = th.grad.clone()
+    th.grad = None
+    e.backward()
+    expect = sim.coef * np.cos(th.detach().numpy())
+    assert np.allclose(g1.numpy(), expect, atol=1e-15)
+    assert np.allclose(th.grad.numpy(), expect, atol=1e-15)
+    assert sim.sweeps == 2
+
+
+def test_autograd_uses_the_prepared_state_when_unchanged():
+    """One forward, one backward: the sweep runs on the state the forward left
+    (no second preparation)."""
+    import torch
+    from frontierlabproject_amd.adjoint import adjoint_expectation
+    sim = _FakeSim([1.0])
+    th = torch.tensor([0.4], dtype=torch.float64, requires_grad=True)
+    adjoint_expectation(th, sim).backward()
+    assert sim.generation == 2 and sim.sweeps == 1  # one forward, one sweep
+
+
+class _FakeDeviceArray:
+    """An object exposing __cuda_array_interface__ (never dereferenced: every
+    check below raises before a device call)."""
+
+    def __init__(self, shape, typestr="<c8", strides=None, ptr=0x1000):
+        self.__cuda_array_interface__ = {"shape": tuple(shape), "typestr": typestr, "data": (ptr, False),
+                                         "strides": strides, "version": 3}
+
+
+@pytest.mark.parametrize("fn", ["apply_1q_gate", "apply_2q_gate"])
+def test_module_gate_functions_refuse_wrong_sizes(hip, fn):
+    """The module functions refuse a device vector whose size or layout does
+    not match 2^num_qubits before launching anything (the kernel would
+    otherwise run past the caller's buffer)."""
+    f = getattr(hip, fn)
+    U = np.eye(2 if fn == "apply_1q_gate" else 4, dtype=np.complex64)
+    args = (0,) if fn == "apply_1q_gate" else (0, 1)
+
+    def call(arr, n):
+        return f(arr, U, *args, n)
+
+    with pytest.raises(RuntimeError, match="2\\^num_qubits elements"):
+        call(_FakeDeviceArray((1 << 10,)), 11)
+    with pytest.raises(RuntimeError, match="2\\^num_qubits elements"):
+        call(_FakeDeviceArray((1 << 12,)), 11)
+    with pytest.raises(RuntimeError, match="contiguous"):
+        call(_FakeDeviceArray((1 << 11,), strides=(16,)), 11)
+    with pytest.raises(RuntimeError, match="null device pointer"):
+        call(_FakeDeviceArray((1 << 11,), ptr=0), 11)
+    with pytest.raises(RuntimeError, match="complex64 or complex128"):
+        call(_FakeDeviceArray((1 << 11,), typestr="<f4"), 11)
+    with pytest.raises(RuntimeError, match="num_qubits must be in"):
+        call(_FakeDeviceArray((1,)), 0)
+    # a 2-D view with the right count and C-contiguous strides passes the
+    # checks; the refusal then comes from the qubit index, still on the host
+    bad = (5,) if fn == "apply_1q_gate" else (0, 5)
+    with pytest.raises(RuntimeError, match="Invalid"):
+        f(_FakeDeviceArray((2, 2), strides=(16, 8)), U, *bad, 2)
+
+
+def test_gate_cache_never_shares_non_scalar_params():
+    """Two same-name gates whose parameters are arrays (or strings) are never
+    served from one cache entry; scalar parameters still are."""
+    from frontierlabproject_amd.backends.custom import backend as B
+
+    class Op:
+        def __init__(self, name, params, M):
+            self.name, self.params, self._M = name, params, M
+
+        def to_matrix(self):
+            return self._M
+
+    m1 = np.diag([1, 1j]).astype(np.complex128)
+    m2 = np.diag([1, -1]).astype(np.complex128)
+    a = Op("diagonal", [np.array([1, 1j])], m1)
+    b = Op("diagonal", [np.array([1, -1])], m2)
+    assert B._gate_key(a, np.complex64) is None and B._gate_key(b, np.complex64) is None
+    assert np.array_equal(B.get_gate_matrix_host(a, np.complex64), m1.astype(np.complex64))
+    assert np.array_equal(B.get_gate_matrix_host(b, np.complex64), m2.astype(np.complex64))
+    assert B._gate_key(Op("pauli", ["XZ"], m1), np.complex64) is None
+    assert B._gate_key(Op("unitary", [m1], m1), np.complex64) is None
+    key = B._gate_key(Op("rz", [0.25], m1), np.complex64)
+    assert key == ("rz", (0.25,), np.dtype(np.complex64).str)
+
+
+def test_usage_merge_rule():
+    """Numbers on both sides: the larger, in the field's CSV format; a number
+    on one side only: that side's text; a first snapshot missing: all empty."""
+    from frontierlabproject_amd.metrics import resources as R
+    empty = {"rss_mb": "", "gpu_mem_mb": "", "gpu_util": ""}
+    assert R._max_usage(None, empty) == empty
+    assert R._max_usage({"rss_mb": "10.00", "gpu_mem_mb": "", "gpu_util": ""},
+                        {"rss_mb": "12.5", "gpu_mem_mb": "3.00", "gpu_util": "7"}) == \
+        {"rss_mb": "12.50", "gpu_mem_mb": "3.00", "gpu_util": "7"}
+    # the first side's number stays as written when the second has none
+    assert R._max_usage({"rss_mb": "3", "gpu_mem_mb": "4.5", "gpu_util": "7"},
+                        {"rss_mb": "", "gpu_mem_mb": "n/a", "gpu_util": "9.6"}) == \
+        {"rss_mb": "3", "gpu_mem_mb": "4.5", "gpu_util": "9"}
+    # a field the second side lacks keeps the first side's text
+    assert R._max_usage({"rss_mb": "", "gpu_mem_mb": "", "gpu_util": ""}, {"rss_mb": "2.00"}) == \
+        {"rss_mb": "2.00", "gpu_mem_mb": "", "gpu_util": ""}
+    snap = R._mem_snapshot()
+    assert set(snap) == {"rss_mb", "gpu_mem_mb", "gpu_util"} and float(snap["rss_mb"]) > 0
