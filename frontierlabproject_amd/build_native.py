@@ -1,7 +1,8 @@
 """Build the HIP shared library in-tree for gfx950.
 
 ``python -m frontierlabproject_amd.build_native`` compiles
-``csrc/*.hip`` and ``csrc/*.cpp`` with hipcc into ``lib/libqgpusim_hip.so``.
+``csrc/*.hip`` and ``csrc/*.cpp`` with hipcc (one process per source, in
+parallel) and links ``lib/libqgpusim_hip.so``.
 All device code is built with packed FP32 instructions disabled.
 """
 import os
@@ -51,25 +52,49 @@ def write_prelude():
             f.write(body)
 
 
+def _run(cmd, verbose):
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    return subprocess.run(cmd, capture_output=True, text=True)
+
+
 def build(verbose=False, extra_flags=(), out=None):
+    """Compile every source to an object in parallel (one hipcc each), then
+    link the shared library; always recompiles."""
+    from concurrent.futures import ThreadPoolExecutor
+
     out = out or LIB
     write_prelude()
     os.makedirs(os.path.dirname(out), exist_ok=True)
-    tmp = out + ".tmp%d" % os.getpid()
-    cmd = [hipcc(), f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-shared",
-           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-           "-Wno-unused-command-line-argument",
-           "-I", os.path.join(ROOT, "include"), "-I", CSRC, "-o", tmp]
-    cmd += list(extra_flags)
-    cmd += ["-x", "hip"] + [os.path.join(CSRC, s) for s in SOURCES]
-    cmd += ["-L", os.path.join(ROCM, "lib"), "-lhiprtc", "-Wl,-rpath," + os.path.join(ROCM, "lib")]
-    if verbose:
-        print(" ".join(cmd), flush=True)
-    res = subprocess.run(cmd, capture_output=True, text=True)
-    if res.returncode != 0:
-        sys.stderr.write(res.stdout + res.stderr)
-        raise RuntimeError("hipcc failed building %s" % out)
-    os.replace(tmp, out)
+    objdir = os.path.join(PKG, "_build", "obj%d" % os.getpid())
+    os.makedirs(objdir, exist_ok=True)
+    common = [hipcc(), f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC",
+              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+              "-Wno-unused-command-line-argument", "-Wno-unused-result",
+              "-I", os.path.join(ROOT, "include"), "-I", CSRC] + list(extra_flags)
+    objs = [os.path.join(objdir, src + ".o") for src in SOURCES]
+
+    def compile_one(i):
+        return _run(common + ["-c", "-x", "hip", os.path.join(CSRC, SOURCES[i]), "-o", objs[i]], verbose)
+
+    try:
+        jobs = min(len(SOURCES), max(1, min(16, os.cpu_count() or 1)))
+        with ThreadPoolExecutor(max_workers=jobs) as ex:
+            results = list(ex.map(compile_one, range(len(SOURCES))))
+        for src, res in zip(SOURCES, results):
+            if res.returncode != 0:
+                sys.stderr.write(res.stdout + res.stderr)
+                raise RuntimeError("hipcc failed compiling %s" % src)
+        tmp = out + ".tmp%d" % os.getpid()
+        link = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs + \
+            ["-L", os.path.join(ROCM, "lib"), "-lhiprtc", "-Wl,-rpath," + os.path.join(ROCM, "lib")]
+        res = _run(link, verbose)
+        if res.returncode != 0:
+            sys.stderr.write(res.stdout + res.stderr)
+            raise RuntimeError("hipcc failed linking %s" % out)
+        os.replace(tmp, out)
+    finally:
+        shutil.rmtree(objdir, ignore_errors=True)
     return out
 
 

@@ -56,6 +56,8 @@ int run_ops(const std::vector<uint64_t> &masks, std::vector<char> &live, uint64_
         const uint64_t m = masks[i];
         if ((m & blocked) || (m & ~M)) {
             blocked |= m;
+            // no later op can run once every chunk bit is blocked
+            if (!(M & ~blocked)) break;
             continue;
         }
         live[i] = 0;
@@ -79,6 +81,8 @@ int greedy_passes(const std::vector<uint64_t> &masks, std::vector<char> live, ui
             const uint64_t m = masks[i];
             if ((m & blocked) || popc(P | m) > K) {
                 blocked |= m;
+                // a full chunk whose bits are all blocked takes no later op
+                if (popc(P) >= K && !(P & ~blocked)) break;
                 continue;
             }
             P |= m;
@@ -90,6 +94,7 @@ int greedy_passes(const std::vector<uint64_t> &masks, std::vector<char> live, ui
 }
 
 constexpr double kSearchMax = 20000;  // chunk-bit sets tried per pass
+constexpr size_t kSearchDeep = 64;    // of which at most this many get the full look-ahead
 // Source bits from log2(2 MiB / amplitude size) up (18 for complex64, 17
 // for complex128) lie 2 MiB apart or more: a chunk gathering over them reads
 // rows from separate address-translation pages.
@@ -101,8 +106,9 @@ inline int far_bit(int elem_bytes) { return elem_bytes == 8 ? 18 : 17; }
 // in-order rule alone can strand an op that opens a third pass), then the one
 // with the fewest bits of `far` (bits the pass reads from far-apart rows: for
 // a folded layout change, those stored high in the old order), then the one
-// running the most ops now.  Returns 0 when the search does not apply (all
-// qubits fit, or too many sets).
+// running the most ops now.  When the sets are many, only the kSearchDeep that
+// run the most ops now get the look-ahead.  Returns 0 when the search does not
+// apply (all qubits fit, or too many sets).
 uint64_t search_chunk(const PlanOp *ops, const std::vector<int> &rem, uint64_t P0, int Kp, uint64_t low, int K,
                       uint64_t far)
 {
@@ -125,8 +131,15 @@ uint64_t search_chunk(const PlanOp *ops, const std::vector<int> &rem, uint64_t P
     int bit[64], nb = 0;
     for (int b = 0; b < 64; ++b)
         if ((Q >> b) & 1) bit[nb++] = b;
-    uint64_t best = 0;
-    int best_rest = 1 << 30, best_far = 1 << 30, best_ran = -1;
+    // every candidate: how many ops it runs now (cheap); the full look-ahead
+    // (passes the in-order rule needs afterwards) only for the best-ranked
+    // kSearchDeep of them when there are more
+    struct Cand {
+        uint64_t M;
+        int ran;
+    };
+    std::vector<Cand> cands;
+    cands.reserve((size_t)combos);
     std::vector<char> live(masks.size());
     for (uint64_t sel = (1ull << free) - 1; sel < (1ull << q);) {
         uint64_t M = P0;
@@ -134,18 +147,28 @@ uint64_t search_chunk(const PlanOp *ops, const std::vector<int> &rem, uint64_t P
             if ((sel >> i) & 1) M |= 1ull << bit[i];
         std::fill(live.begin(), live.end(), 1);
         const int ran = run_ops(masks, live, M);
-        if (ran > 0) {
-            const int rest = greedy_passes(masks, live, low, K);
-            const int nfar = popc(M & far);
-            if (rest < best_rest || (rest == best_rest && (nfar < best_far || (nfar == best_far && ran > best_ran)))) {
-                best = M;
-                best_rest = rest;
-                best_far = nfar;
-                best_ran = ran;
-            }
-        }
+        if (ran > 0) cands.push_back({M, ran});
         const uint64_t c = sel & (~sel + 1), r = sel + c;  // next set of the same size
         sel = (((r ^ sel) >> 2) / c) | r;
+    }
+    if (cands.size() > kSearchDeep)
+        std::stable_sort(cands.begin(), cands.end(), [](const Cand &a, const Cand &b) { return a.ran > b.ran; });
+    uint64_t best = 0;
+    int best_rest = 1 << 30, best_far = 1 << 30, best_ran = -1;
+    const size_t deep = std::min(cands.size(), kSearchDeep);
+    for (size_t k = 0; k < deep; ++k) {
+        const uint64_t M = cands[k].M;
+        const int ran = cands[k].ran;
+        std::fill(live.begin(), live.end(), 1);
+        run_ops(masks, live, M);
+        const int rest = greedy_passes(masks, live, low, K);
+        const int nfar = popc(M & far);
+        if (rest < best_rest || (rest == best_rest && (nfar < best_far || (nfar == best_far && ran > best_ran)))) {
+            best = M;
+            best_rest = rest;
+            best_far = nfar;
+            best_ran = ran;
+        }
     }
     return best;
 }
