@@ -98,6 +98,16 @@ def _load():
         raise ImportError(
             "HIP backend library %s is not built: run "
             "`python -m frontierlabproject_amd.build_native` (hipcc, gfx950)" % LIB_PATH)
+    # PyTorch ships its own HIP runtime, and it finds no device when the
+    # system runtime this library links has initialised the GPU first in the
+    # same process (the reverse order works).  Loading torch first keeps
+    # torch usable beside the library (device tensors for the module gate
+    # functions, autograd); QGPUSIM_NO_TORCH=1 skips it.
+    if not os.environ.get("QGPUSIM_NO_TORCH"):
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
     lib = ctypes.CDLL(LIB_PATH)
     for name, (args, res) in _SIGNATURES.items():
         fn = getattr(lib, name)

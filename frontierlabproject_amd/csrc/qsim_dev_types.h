@@ -82,6 +82,11 @@ enum : int32_t {
     S1_REAL1 = 14,     // [[1, a],[b, 1]], a b real               1 cx (a,b)
     S1_RI1 = 15,       // [[1, iq],[ir, 1]], q r real             1 cx (q,r)
     S1_GEN1 = 16,      // [[1, b],[c, d]]  (u0 complex, taken out) 3 cx
+    S1_NOP = 17,       // identity (a diagonal gate reduced to its factor)
+    S1_REALG = 18,     // [[1, a],[b, d]], a b d real (u0 taken out) 2 cx (a,b),(d,0)
+    // flag: exchange the pair (x, y) first, then apply the kind (U = V X with
+    // V = U X; a register rename in specialised kernels)
+    S1_SWAP = 64,
 };
 // pair slots (matrix index b_first + 2 b_second, first < second register bit);
 // other pair ops make the step dense:

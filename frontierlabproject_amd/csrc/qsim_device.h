@@ -106,10 +106,41 @@ __device__ __forceinline__ double wave_sum(double v)
     return v;
 }
 
+// A wave-uniform 64-bit value pinned to scalar registers, so that a lane
+// offset added to a pointer plus it becomes the scalar-base + 32-bit-offset
+// form of a global access (no 64-bit address arithmetic per access on the
+// vector unit).
+__device__ __forceinline__ uint64_t uniform_u64(uint64_t a)
+{
+    // an empty asm statement (no instruction) that takes and returns the
+    // value in scalar registers: the optimiser can no longer split the sum
+    // it came from across the vector address arithmetic
+    asm volatile("" : "+s"(a));
+    return a;
+}
+
+// LDS element at byte offset `byte` of `buf` (callers keep offsets in bytes
+// so that xors with constant offsets need no scaling).
+template <typename R>
+__device__ __forceinline__ cx<R> &lds_at(cx<R> *buf, int byte)
+{
+    return *reinterpret_cast<cx<R> *>(reinterpret_cast<char *>(buf) + byte);
+}
+
 // 1q slot arithmetic on register bit B (applied to one vector).
 template <typename R, int B>
 __device__ __forceinline__ void apply1(cx<R> *a, int kind, const cx<R> *__restrict__ mp)
 {
+    if (kind & S1_SWAP) {  // U = V X: exchange each pair, then apply V
+#pragma unroll
+        for (int m = 0; m < 16; ++m)
+            if (!((m >> B) & 1)) {
+                const cx<R> t = a[m];
+                a[m] = a[m | (1 << B)];
+                a[m | (1 << B)] = t;
+            }
+        kind &= ~S1_SWAP;
+    }
     switch (kind) {
     case S1_GEN: {
         const cx<R> u[4] = {mp[0], mp[1], mp[2], mp[3]};
@@ -174,6 +205,18 @@ __device__ __forceinline__ void apply1(cx<R> *a, int kind, const cx<R> *__restri
                 const cx<R> x = a[m], y = a[m1];
                 a[m] = cx<R>{fma(-qr.re, y.im, x.re), fma(qr.re, y.re, x.im)};
                 a[m1] = cx<R>{fma(-qr.im, x.im, y.re), fma(qr.im, x.re, y.im)};
+            }
+        break;
+    }
+    case S1_REALG: {  // x' = x + a y, y' = b x + d y (a, b, d real)
+        const cx<R> ab = mp[0], d0 = mp[1];
+#pragma unroll
+        for (int m = 0; m < 16; ++m)
+            if (!((m >> B) & 1)) {
+                const int m1 = m | (1 << B);
+                const cx<R> x = a[m], y = a[m1];
+                a[m] = cx<R>{fma(ab.re, y.re, x.re), fma(ab.re, y.im, x.im)};
+                a[m1] = cx<R>{fma(ab.im, x.re, d0.re * y.re), fma(ab.im, x.im, d0.re * y.im)};
             }
         break;
     }

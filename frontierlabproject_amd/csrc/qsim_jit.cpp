@@ -44,7 +44,10 @@ bool skip_pass(const Plan &plan, size_t p, int nv) { return nv == 1 && plan.pass
 constexpr int kPairF[6] = {0, 2, 0, 1, 0, 1};
 constexpr int kPairS[6] = {1, 3, 2, 3, 3, 2};
 
-void emit_step(std::ostringstream &o, const StepDev &sd, int E, const char *T, int nv, int tpb)
+// lut >= 0: the step's base slot is L<lut> (the lane part, read from the
+// kernel's lane table) xor its wave part, instead of being computed per
+// thread (needs one register group per thread: NG == tpb).
+void emit_step(std::ostringstream &o, const StepDev &sd, int E, const char *T, int nv, int tpb, int lut = -1)
 {
     const int NG = E >> kG;
     int moff[16];
@@ -74,13 +77,26 @@ void emit_step(std::ostringstream &o, const StepDev &sd, int E, const char *T, i
     // 16-byte LDS accesses
     bool bit0 = false;
     for (int q = 0; q < 4; ++q) bit0 |= sd.bits[q] == 0;
-    o << "    const int sb = swz((int)ins0(ins0(ins0(ins0(g32, " << sd.sbits[0] << "), " << sd.sbits[1] << "), "
-      << sd.sbits[2] << "), " << sd.sbits[3] << "))" << (bit0 ? " & ~1" : "") << ";\n";
+    // LDS addressing in bytes (slot * amplitude size): the xor with each
+    // register offset then needs no shift
+    const int esz = std::string(T) == "float" ? 8 : 16;
+    if (lut >= 0 && !loop && !guard) {
+        // lane part from the kernel's table (bytes), the wave part (uniform)
+        // on the scalar unit: swz and the zero insertion are linear
+        o << "    const int sb = (L" << lut << " ^ (swz((int)ins0(ins0(ins0(ins0(wv << 6, " << sd.sbits[0] << "), "
+          << sd.sbits[1] << "), " << sd.sbits[2] << "), " << sd.sbits[3] << ")) * " << esz << "))"
+          << (bit0 ? " & ~" + std::to_string(esz) : std::string()) << ";\n";
+    } else {
+        o << "    const int sb = (swz((int)ins0(ins0(ins0(ins0(g32, " << sd.sbits[0] << "), " << sd.sbits[1] << "), "
+          << sd.sbits[2] << "), " << sd.sbits[3] << "))" << (bit0 ? " & ~1" : "") << ") * " << esz << ";\n";
+    }
+    for (int m = 0; m < 16; ++m) moff[m] *= esz;
     // vectors: a (psi, in c0) and, with nv == 2, l (lam, in c1)
     const char *vec[2] = {"a", "l"}, *lds[2] = {"c0", "c1"};
     for (int v = 0; v < nv; ++v) {
         o << "    cx<" << T << "> " << vec[v] << "[16];\n";
-        for (int m = 0; m < 16; ++m) o << "    " << vec[v] << "[" << m << "] = " << lds[v] << "[sb ^ " << moff[m] << "];\n";
+        for (int m = 0; m < 16; ++m)
+            o << "    " << vec[v] << "[" << m << "] = lds_at(" << lds[v] << ", sb ^ " << moff[m] << ");\n";
     }
     const char *store_guard = guard ? "if (act) " : "";
     if (sd.dense >= 0) {
@@ -92,7 +108,7 @@ void emit_step(std::ostringstream &o, const StepDev &sd, int E, const char *T, i
                 o << "      #pragma unroll\n      for (int c = 1; c < 16; ++c) acc = cmad(mats[" << M
                   << " + c], " << vec[v] << "[c], acc);\n";
                 if (sd.scale >= 0) o << "      acc = cmul(acc, f);\n";
-                o << "      " << store_guard << lds[v] << "[sb ^ " << moff[r] << "] = acc; }\n";
+                o << "      " << store_guard << "lds_at(" << lds[v] << ", sb ^ " << moff[r] << ") = acc; }\n";
             }
     } else {
         for (int r = 0; r < sd.nrounds; ++r) {
@@ -136,7 +152,8 @@ void emit_step(std::ostringstream &o, const StepDev &sd, int E, const char *T, i
         }
         for (int v = 0; v < nv; ++v)
             for (int m = 0; m < 16; ++m)
-                o << "    " << store_guard << lds[v] << "[sb ^ " << moff[m] << "] = " << vec[v] << "[" << m << "];\n";
+                o << "    " << store_guard << "lds_at(" << lds[v] << ", sb ^ " << moff[m] << ") = " << vec[v] << "[" << m
+                  << "];\n";
     }
     if (loop) o << "  }\n";
     o << "  }\n  __syncthreads();\n";
@@ -250,6 +267,163 @@ std::string bit_moves(const char *x, const int *dst, int nbits, const char *ty)
     return o.str();
 }
 
+// Per-lane constants of a pass kernel, computed here and compiled into the
+// kernel as a table (one 64-entry row per item, read once per thread at the
+// start): the lane part of each copy's offset and LDS slot, and of each
+// register step's base slot.  Offsets, slots and the zero insertion of the
+// steps are linear over GF(2) and the lane and wave parts of the thread index
+// have disjoint bits, so the wave part (uniform) is added on the scalar unit.
+struct LaneTable {
+    std::vector<int32_t> v;  // rows of 64
+    int add_row(const std::vector<int32_t> &r)
+    {
+        v.insert(v.end(), r.begin(), r.end());
+        return (int)(v.size() / 64) - 1;
+    }
+};
+
+uint64_t deposit(uint64_t x, const int *to, int nbits)
+{
+    uint64_t r = 0;
+    for (int k = 0; k < nbits; ++k)
+        if ((x >> k) & 1) r |= 1ull << to[k];
+    return r;
+}
+
+uint32_t ins0_host(uint32_t x, int p) { return ((x >> p) << (p + 1)) | (x & ((1u << p) - 1u)); }
+
+// Row of the step's lane parts, or -1 (the step computes its base per
+// thread: several groups per thread, or a dense matrix-core step).
+int step_row(LaneTable &lt, const StepDev &sd, int E, const char *T, int nv, int tpb)
+{
+    const bool mfma = sd.dense >= 0 && nv == 1 && std::string(T) == "float";
+    if ((E >> kG) != tpb || mfma) return -1;
+    const int esz = std::string(T) == "float" ? 8 : 16;
+    std::vector<int32_t> r(64);
+    for (uint32_t l = 0; l < 64; ++l)
+        r[l] = esz * swz_host((int)ins0_host(ins0_host(ins0_host(ins0_host(l, sd.sbits[0]), sd.sbits[1]),
+                                                       sd.sbits[2]), sd.sbits[3]));
+    return lt.add_row(r);
+}
+
+// Specialised chunk copies.  The read index r of a chunk element splits into
+// a lane part (bits below log2(per * 64); per = 2 complex64 amplitudes per
+// 16-byte access, 1 complex128), a wave part (bits below log2(per * TPB))
+// and a batch part j * per * TPB; each access is then the scalar address
+// (base + C_j + wave part) plus the lane's 32-bit byte offset, into LDS slot
+// (lane slot ^ wave slot ^ S_j), with C_j and S_j compile-time constants.
+// src[k]: state position of read bit k; chunk[k]: chunk bit of read bit k
+// (the identity unless a layout change is folded in).
+struct CopyPlan {
+    bool ok = false;
+    int per = 1, tb = 0, lb = 0, nj = 0;
+    int row = -1;  // lane table rows: offsets (bytes) at row, slots at row + 1
+};
+
+CopyPlan copy_plan(LaneTable &lt, int K, bool dbl, int tpb, const int *src, const int *chunk, int cf)
+{
+    CopyPlan c;
+    c.per = dbl ? 1 : 2;
+    int lt2 = 0;
+    while ((1 << lt2) < tpb) ++lt2;
+    c.tb = lt2 + (c.per == 2 ? 1 : 0);
+    c.lb = 6 + (c.per == 2 ? 1 : 0);
+    if (K < c.tb || (!dbl && cf < 1)) return c;  // guards, or unpaired complex64 reads
+    int hi = 0;
+    for (int k = 0; k < c.lb; ++k) hi = std::max(hi, src[k]);
+    if (hi + 5 >= 31) return c;  // lane offsets must fit in 32 bits
+    c.nj = 1 << (K - c.tb);
+    std::vector<int32_t> off(64), slot(64);
+    const int esz = dbl ? 16 : 8;
+    for (uint32_t l = 0; l < 64; ++l) {
+        const uint64_t r = (uint64_t)l * c.per;
+        off[l] = (int32_t)(deposit(r, src, c.lb) * esz);
+        slot[l] = swz_host((int)deposit(r, chunk, c.lb)) * esz;  // LDS bytes
+    }
+    c.row = lt.add_row(off);
+    lt.add_row(slot);
+    c.ok = true;
+    return c;
+}
+
+// Declares the thread's lane offset Tl<tag>, wave offset Tw<tag> (scalar,
+// amplitudes) and slot st<tag>.
+void emit_copy_thread(std::ostringstream &o, const CopyPlan &c, const char *tag, const int *src, const int *chunk)
+{
+    const std::string t(tag);
+    o << "  const uint32_t Tl" << t << " = (uint32_t)qtab[" << c.row * 64 << " + lane];\n";
+    o << "  const uint32_t rw" << t << " = wv << " << c.lb << ";\n";
+    o << "  const uint64_t Tw" << t << " = 0ull" << bit_moves(("rw" + t).c_str(), src, c.tb, "uint64_t") << ";\n";
+    o << "  const int st" << t << " = qtab[" << (c.row + 1) * 64 << " + lane] ^ (swz(0"
+      << bit_moves(("rw" + t).c_str(), chunk, c.tb, "int") << ") * " << (c.per == 2 ? 8 : 16) << ");\n";
+}
+
+// Copy between global vector `g` and LDS `lds` over the K read bits: loads
+// (global -> LDS, `chunk` may reorder the bits: a folded layout change) or
+// stores (LDS -> global, the pass layout).  Accesses are issued in batches of
+// eight per thread, all loads of a batch before any of its LDS writes.
+void emit_copy(std::ostringstream &o, const CopyPlan &c, const char *tag, const char *g, const char *lds,
+               const char *base, const int *src, const int *chunk, int K, bool load, bool dbl)
+{
+    const int B = 8;
+    const std::string t(tag);
+    const char *T = dbl ? "double" : "float";
+    bool ident = true;
+    for (int k = 0; k < K; ++k) ident &= chunk[k] == k;
+    const int esz = dbl ? 16 : 8;
+    const int s1 = swz_host(1 << chunk[0]) * esz;  // complex64: a pair's second amplitude (LDS bytes, xor)
+    for (int j0 = 0; j0 < c.nj; j0 += B) {
+        const int nb = std::min(B, c.nj - j0);
+        o << "  {\n    v4f b[" << nb << "];\n";
+        for (int pass = 0; pass < 2; ++pass)
+            for (int q = 0; q < nb; ++q) {
+                const uint64_t rj = (uint64_t)(j0 + q) << c.tb;
+                const uint64_t Cj = deposit(rj, src, K);
+                const int Sj = swz_host((int)deposit(rj, chunk, K)) * esz;
+                std::ostringstream a;
+                a << "reinterpret_cast<v4f *>(reinterpret_cast<char *>(const_cast<cx<" << T << "> *>(" << g
+                  << ") + uniform_u64(" << base << " + Tw" << t << " + " << Cj << "ull)) + Tl" << t << ")";
+                const std::string slot = "st" + t + " ^ " + std::to_string(Sj);
+                const std::string la = "reinterpret_cast<v4f *>(reinterpret_cast<char *>(" + std::string(lds) + ") + (" +
+                                       slot + "))";
+                if (load && pass == 0) {
+                    o << "    b[" << q << "] = __builtin_nontemporal_load(" << a.str() << ");\n";
+                } else if (load) {
+                    if (dbl || ident)
+                        o << "    *" << la << " = b[" << q << "];\n";
+                    else
+                        o << "    { const int sl = " << slot << "; lds_at(" << lds << ", sl) = cx<float>{b[" << q
+                          << "].x, b[" << q << "].y}; lds_at(" << lds << ", sl ^ " << s1 << ") = cx<float>{b[" << q
+                          << "].z, b[" << q << "].w}; }\n";
+                } else if (pass == 0) {
+                    o << "    b[" << q << "] = *" << la << ";\n";
+                } else {
+                    o << "    __builtin_nontemporal_store(b[" << q << "], " << a.str() << ");\n";
+                }
+            }
+        o << "  }\n";
+    }
+}
+
+// The lane table as a device array (named after the kernel) and the per-thread
+// lane / wave indices.
+std::string lane_table_source(const LaneTable &lt, const std::string &name)
+{
+    std::ostringstream o;
+    o << "static __device__ const int32_t " << name << "_t[" << std::max<size_t>(1, lt.v.size()) << "] = {";
+    for (size_t i = 0; i < lt.v.size(); ++i) o << (i ? "," : "") << lt.v[i];
+    if (lt.v.empty()) o << "0";
+    o << "};\n";
+    return o.str();
+}
+
+void emit_lane_prologue(std::ostringstream &o, const std::string &name)
+{
+    o << "  const int32_t *__restrict__ qtab = " << name << "_t;\n";
+    o << "  const int lane = (int)(threadIdx.x & 63);\n";
+    o << "  const uint32_t wv = (uint32_t)__builtin_amdgcn_readfirstlane((int)threadIdx.x) >> 6;\n";
+}
+
 // First pass of a plan with a folded layout change: reads `psi` in the
 // source layout, writes `out` in the pass layout (complex64, one vector).
 std::string mapped_pass_source(const Plan &plan, bool dbl, const std::string &name)
@@ -277,6 +451,28 @@ std::string mapped_pass_source(const Plan &plan, bool dbl, const std::string &na
       << (dbl ? "" : " __attribute__((amdgpu_waves_per_eu(4)))") << "\nvoid " << name << "(const cx<" << T
       << "> *__restrict__ psi, cx<" << T << "> *__restrict__ out, const cx<" << T << "> *__restrict__ mats)\n{\n";
     o << "  __shared__ __align__(16) cx<" << T << "> c0[" << E << "];\n";
+    int ident[64];
+    for (int k = 0; k < 64; ++k) ident[k] = k;
+    LaneTable lt;
+    const CopyPlan ci = copy_plan(lt, K, dbl, TPB, Pold, J, CFR), co = copy_plan(lt, K, dbl, TPB, pd.pos, ident, CF);
+    if (ci.ok && co.ok) {
+        std::vector<int> rows(pd.nsteps);
+        for (int s = 0; s < pd.nsteps; ++s) rows[s] = step_row(lt, plan.steps[pd.step_off + s], E, T, 1, TPB);
+        std::ostringstream b;
+        emit_lane_prologue(b, name);
+        for (int r : rows)
+            if (r >= 0) b << "  const int L" << r << " = qtab[" << r * 64 << " + lane];\n";
+        b << "  const uint64_t c = xcd_slab(blockIdx.x, " << pd.ncomp << ");\n";
+        b << "  const uint64_t base = 0ull" << bit_moves("c", pd.comp, pd.ncomp, "uint64_t") << ";\n";
+        b << "  const uint64_t base_in = 0ull" << bit_moves("c", comp_old, pd.ncomp, "uint64_t") << ";\n";
+        emit_copy_thread(b, ci, "i", Pold, J);
+        emit_copy_thread(b, co, "o", pd.pos, ident);
+        emit_copy(b, ci, "i", "psi", "c0", "base_in", Pold, J, K, true, dbl);
+        b << "  __syncthreads();\n";
+        for (int s = 0; s < pd.nsteps; ++s) emit_step(b, plan.steps[pd.step_off + s], E, T, 1, TPB, rows[s]);
+        emit_copy(b, co, "o", "out", "c0", "base", pd.pos, ident, K, false, dbl);
+        return lane_table_source(lt, name) + o.str() + b.str() + "}\n\n";
+    }
     o << "  __shared__ uint64_t rowoff[" << NROW << "];\n  __shared__ uint64_t rowin[" << NROWIN << "];\n";
     o << "  for (int t = threadIdx.x; t < " << NROW << "; t += " << TPB << ")\n    rowoff[t] = 0ull"
       << bit_moves("t", pd.pos + CF, K - CF, "uint64_t") << ";\n";
@@ -332,6 +528,42 @@ std::string jit_pass_source(const Plan &plan, size_t p, bool dbl, int nv, const 
         o << "  __shared__ __align__(16) cx<" << T << "> c1[" << E << "];\n";
         o << "  __shared__ double pacc[" << NP << " * kWaves];\n";
         o << "  for (int t = threadIdx.x; t < " << NP << " * kWaves; t += kTPB) pacc[t] = 0.0;\n";
+    }
+    int ident[64];
+    for (int k = 0; k < 64; ++k) ident[k] = k;
+    LaneTable lt;
+    const CopyPlan cp = copy_plan(lt, K, dbl, TPB, pd.pos, ident, CF);
+    if (cp.ok) {
+        std::vector<int> rows(pd.nsteps);
+        for (int s = 0; s < pd.nsteps; ++s) rows[s] = step_row(lt, plan.steps[pd.step_off + s], E, T, nv, TPB);
+        std::ostringstream b;
+        emit_lane_prologue(b, name);
+        for (int r : rows)
+            if (r >= 0) b << "  const int L" << r << " = qtab[" << r * 64 << " + lane];\n";
+        b << "  const uint64_t c = xcd_slab(blockIdx.x, " << pd.ncomp << ");\n";
+        b << "  const uint64_t base = 0ull" << bit_moves("c", pd.comp, pd.ncomp, "uint64_t") << ";\n";
+        emit_copy_thread(b, cp, "", pd.pos, ident);
+        if (p == 0 && plan.zero_input && nv == 1) {
+            // the input is |0...0>: the chunk holding amplitude 0 starts as e_0, every other as zeros
+            b << "  for (int t = threadIdx.x; t < " << E << "; t += " << TPB << ") c0[t] = cx<" << T << ">{0, 0};\n";
+            b << "  __syncthreads();\n";
+            b << "  if (base == 0 && threadIdx.x == 0) c0[swz(0)] = cx<" << T << ">{1, 0};\n";
+        } else {
+            emit_copy(b, cp, "", "psi", "c0", "base", pd.pos, ident, K, true, dbl);
+        }
+        if (nv == 2) emit_copy(b, cp, "", "lam", "c1", "base", pd.pos, ident, K, true, dbl);
+        b << "  __syncthreads();\n";
+        for (int s = 0; s < pd.nsteps; ++s) emit_step(b, plan.steps[pd.step_off + s], E, T, nv, TPB, rows[s]);
+        emit_copy(b, cp, "", "psi", "c0", "base", pd.pos, ident, K, false, dbl);
+        if (nv == 2) {
+            emit_copy(b, cp, "", "lam", "c1", "base", pd.pos, ident, K, false, dbl);
+            if (pd.nprobe > 0) {
+                b << "  for (int q = threadIdx.x; q < " << pd.nprobe << "; q += kTPB) {\n";
+                b << "    double v = 0.0;\n    for (int w = 0; w < kWaves; ++w) v += pacc[q * kWaves + w];\n";
+                b << "    probe_partial[c * " << pd.nprobe << " + q] = v;\n  }\n";
+            }
+        }
+        return lane_table_source(lt, name) + o.str() + b.str() + "}\n\n";
     }
     o << "  __shared__ uint64_t rowoff[" << NROW << "];\n";
     o << "  for (int t = threadIdx.x; t < " << NROW << "; t += " << TPB << ") {\n    uint64_t r = 0;\n";

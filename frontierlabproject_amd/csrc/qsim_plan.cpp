@@ -254,8 +254,6 @@ int push_mats(Plan &p, const Cd *v, int count)
     return off;
 }
 
-// Place op `idx` (register bits rx, ry already assigned) into the step.
-// Returns 1 when placed, 0 when no slot is free, 2 when the op needs a dense step.
 // u0 can be taken out of a 1q matrix without growing its other entries:
 // |u0| is at least |u1| and |u2| (at least 1/sqrt2 for a unitary).
 inline bool factorable(const Cd *u)
@@ -264,6 +262,101 @@ inline bool factorable(const Cd *u)
     return a > 0.0 && a >= u[1].re * u[1].re + u[1].im * u[1].im && a >= u[2].re * u[2].re + u[2].im * u[2].im;
 }
 
+// Slot form of the 1q gate u in slot j (not a probe): each gate is
+// classified exactly (entries compared with 0, 1, i) and runs the cheapest
+// arithmetic for its form; scalars taken out go to the step's factor.
+void classify_1q(Ctx &c, StepBuilder &sb, const Cd *u, int j)
+{
+    Plan &p = *c.plan;
+    if (is0(u[1]) && is0(u[2]) &&
+        (c.real_scale ? !is0(u[0]) : std::fabs(u[0].re * u[0].re + u[0].im * u[0].im - 1.0) < 1e-12)) {
+        // diag(u0, u3) = u0 * diag(1, u3/u0): the phase u0 is applied later
+        const Cd d = cdivh(u[3], u[0]);
+        sb.factor = cmulh(sb.factor, u[0]);
+        if (iseq(d, 1, 0)) {
+            sb.sd.kind[j] = S1_NOP;
+        } else if (iseq(d, -1, 0)) {
+            sb.sd.kind[j] = S1_Z;
+            sb.cost += 0.5;
+        } else if (iseq(d, 0, 1)) {
+            sb.sd.kind[j] = S1_PI;
+            sb.cost += 0.5;
+        } else if (iseq(d, 0, -1)) {
+            sb.sd.kind[j] = S1_MI;
+            sb.cost += 0.5;
+        } else {
+            sb.sd.kind[j] = S1_PHASE;
+            sb.sd.mat[j] = push_mats(p, &d, 1);
+            sb.cost += 2.0;
+        }
+    } else if (is0(u[1]) && is0(u[2])) {
+        Cd d[2] = {u[0], u[3]};
+        sb.sd.kind[j] = S1_DIAG;
+        sb.sd.mat[j] = push_mats(p, d, 2);
+        sb.cost += 4.0;
+    } else if (is0(u[0]) && is0(u[3]) && is1(u[1]) && is1(u[2])) {
+        sb.sd.kind[j] = S1_X;
+        sb.cost += 0.5;
+    } else if (is0(u[0]) && is0(u[3]) && iseq(u[1], 0, -1) && iseq(u[2], 0, 1)) {
+        sb.sd.kind[j] = S1_Y;
+        sb.cost += 1.0;
+    } else if (c.real_scale && isre(u[0]) && u[0].re != 0.0 && iseq(u[1], u[0].re, 0) &&
+               iseq(u[2], u[0].re, 0) && iseq(u[3], -u[0].re, 0)) {
+        // s * [[1, 1], [1, -1]] (H): butterfly, s applied later
+        sb.factor = cmulh(sb.factor, u[0]);
+        sb.sd.kind[j] = S1_BFLY;
+        sb.cost += 2.0;
+    } else if (isre(u[0]) && isre(u[1]) && isre(u[2]) && isre(u[3])) {
+        if (c.real_scale && u[0].re == u[3].re && factorable(u)) {
+            // u0 * [[1, a], [b, 1]] (RY): u0 applied later
+            Cd ab = {u[1].re / u[0].re, u[2].re / u[0].re};
+            sb.factor = cmulh(sb.factor, u[0]);
+            sb.sd.kind[j] = S1_REAL1;
+            sb.cost += 2.0;
+            sb.sd.mat[j] = push_mats(p, &ab, 1);
+        } else if (c.real_scale && factorable(u)) {
+            // u0 * [[1, a], [b, d]], all real: u0 applied later
+            Cd abd[2] = {{u[1].re / u[0].re, u[2].re / u[0].re}, {u[3].re / u[0].re, 0.0}};
+            sb.factor = cmulh(sb.factor, u[0]);
+            sb.sd.kind[j] = S1_REALG;
+            sb.cost += 3.0;
+            sb.sd.mat[j] = push_mats(p, abd, 2);
+        } else {
+            Cd r[2] = {{u[0].re, u[1].re}, {u[2].re, u[3].re}};
+            sb.sd.kind[j] = S1_REAL;
+            sb.cost += 4.0;
+            sb.sd.mat[j] = push_mats(p, r, 2);
+        }
+    } else if (isre(u[0]) && isre(u[3]) && isim(u[1]) && isim(u[2])) {
+        if (c.real_scale && u[0].re == u[3].re && factorable(u)) {
+            // u0 * [[1, iq], [ir, 1]] (RX): u0 applied later
+            Cd qr = {u[1].im / u[0].re, u[2].im / u[0].re};
+            sb.factor = cmulh(sb.factor, u[0]);
+            sb.sd.kind[j] = S1_RI1;
+            sb.cost += 2.0;
+            sb.sd.mat[j] = push_mats(p, &qr, 1);
+        } else {
+            Cd r[2] = {{u[0].re, u[1].im}, {u[2].im, u[3].re}};
+            sb.sd.kind[j] = S1_RI;
+            sb.cost += 4.0;
+            sb.sd.mat[j] = push_mats(p, r, 2);
+        }
+    } else if (c.real_scale && factorable(u)) {
+        // u0 * [[1, b], [c, d]]: u0 applied later
+        Cd g[3] = {cdivh(u[1], u[0]), cdivh(u[2], u[0]), cdivh(u[3], u[0])};
+        sb.factor = cmulh(sb.factor, u[0]);
+        sb.sd.kind[j] = S1_GEN1;
+        sb.cost += 6.0;
+        sb.sd.mat[j] = push_mats(p, g, 3);
+    } else {
+        sb.sd.kind[j] = S1_GEN;
+        sb.cost += 8.0;
+        sb.sd.mat[j] = push_mats(p, u, 4);
+    }
+}
+
+// Place op `idx` (register bits rx, ry already assigned) into the step.
+// Returns 1 when placed, 0 when no slot is free, 2 when the op needs a dense step.
 int place(Ctx &c, StepBuilder &sb, int idx, int rx, int ry)
 {
     const PlanOp &op = c.ops[idx];
@@ -296,81 +389,16 @@ int place(Ctx &c, StepBuilder &sb, int idx, int rx, int ry)
             sb.sd.mat[j] = push_mats(p, g8, 8);
             sb.sd.probe[(j / kSlotsPerRound) * 4 + t] = c.pd->nprobe++;
             p.probe_ids.push_back(op.probe);
-        } else if (is0(u[1]) && is0(u[2]) &&
-                   (c.real_scale ? !is0(u[0]) : std::fabs(u[0].re * u[0].re + u[0].im * u[0].im - 1.0) < 1e-12)) {
-            // diag(u0, u3) = u0 * diag(1, u3/u0): the phase u0 is applied later
-            const Cd d = cdivh(u[3], u[0]);
-            sb.factor = cmulh(sb.factor, u[0]);
-            if (iseq(d, -1, 0)) {
-                sb.sd.kind[j] = S1_Z;
-                sb.cost += 0.5;
-            } else if (iseq(d, 0, 1)) {
-                sb.sd.kind[j] = S1_PI;
-                sb.cost += 0.5;
-            } else if (iseq(d, 0, -1)) {
-                sb.sd.kind[j] = S1_MI;
-                sb.cost += 0.5;
-            } else {
-                sb.sd.kind[j] = S1_PHASE;
-                sb.sd.mat[j] = push_mats(p, &d, 1);
-                sb.cost += 2.0;
-            }
-        } else if (is0(u[1]) && is0(u[2])) {
-            Cd d[2] = {u[0], u[3]};
-            sb.sd.kind[j] = S1_DIAG;
-            sb.sd.mat[j] = push_mats(p, d, 2);
-            sb.cost += 4.0;
-        } else if (is0(u[0]) && is0(u[3]) && is1(u[1]) && is1(u[2])) {
-            sb.sd.kind[j] = S1_X;
-            sb.cost += 0.5;
-        } else if (is0(u[0]) && is0(u[3]) && iseq(u[1], 0, -1) && iseq(u[2], 0, 1)) {
-            sb.sd.kind[j] = S1_Y;
-            sb.cost += 1.0;
-        } else if (c.real_scale && isre(u[0]) && u[0].re != 0.0 && iseq(u[1], u[0].re, 0) &&
-                   iseq(u[2], u[0].re, 0) && iseq(u[3], -u[0].re, 0)) {
-            // s * [[1, 1], [1, -1]] (H): butterfly, s applied later
-            sb.factor = cmulh(sb.factor, u[0]);
-            sb.sd.kind[j] = S1_BFLY;
-            sb.cost += 2.0;
-        } else if (isre(u[0]) && isre(u[1]) && isre(u[2]) && isre(u[3])) {
-            if (c.real_scale && u[0].re == u[3].re && factorable(u)) {
-                // u0 * [[1, a], [b, 1]] (RY): u0 applied later
-                Cd ab = {u[1].re / u[0].re, u[2].re / u[0].re};
-                sb.factor = cmulh(sb.factor, u[0]);
-                sb.sd.kind[j] = S1_REAL1;
-                sb.cost += 2.0;
-                sb.sd.mat[j] = push_mats(p, &ab, 1);
-            } else {
-                Cd r[2] = {{u[0].re, u[1].re}, {u[2].re, u[3].re}};
-                sb.sd.kind[j] = S1_REAL;
-                sb.cost += 4.0;
-                sb.sd.mat[j] = push_mats(p, r, 2);
-            }
-        } else if (isre(u[0]) && isre(u[3]) && isim(u[1]) && isim(u[2])) {
-            if (c.real_scale && u[0].re == u[3].re && factorable(u)) {
-                // u0 * [[1, iq], [ir, 1]] (RX): u0 applied later
-                Cd qr = {u[1].im / u[0].re, u[2].im / u[0].re};
-                sb.factor = cmulh(sb.factor, u[0]);
-                sb.sd.kind[j] = S1_RI1;
-                sb.cost += 2.0;
-                sb.sd.mat[j] = push_mats(p, &qr, 1);
-            } else {
-                Cd r[2] = {{u[0].re, u[1].im}, {u[2].im, u[3].re}};
-                sb.sd.kind[j] = S1_RI;
-                sb.cost += 4.0;
-                sb.sd.mat[j] = push_mats(p, r, 2);
-            }
-        } else if (c.real_scale && factorable(u)) {
-            // u0 * [[1, b], [c, d]]: u0 applied later
-            Cd g[3] = {cdivh(u[1], u[0]), cdivh(u[2], u[0]), cdivh(u[3], u[0])};
-            sb.factor = cmulh(sb.factor, u[0]);
-            sb.sd.kind[j] = S1_GEN1;
-            sb.cost += 6.0;
-            sb.sd.mat[j] = push_mats(p, g, 3);
         } else {
-            sb.sd.kind[j] = S1_GEN;
-            sb.cost += 8.0;
-            sb.sd.mat[j] = push_mats(p, u, 4);
+            // With u0 not the largest entry of its row, U = V X with V = U X
+            // (columns exchanged), whose u0 is: the slot exchanges the pair
+            // (a register rename in specialised kernels) and runs V's form.
+            const Cd ux[4] = {u[1], u[0], u[3], u[2]};
+            const bool swap = c.real_scale && !(is0(u[1]) && is0(u[2])) && !factorable(u) && factorable(ux) &&
+                              !(is0(u[0]) && is0(u[3]) && ((is1(u[1]) && is1(u[2])) ||
+                                                            (iseq(u[1], 0, -1) && iseq(u[2], 0, 1))));
+            classify_1q(c, sb, swap ? ux : u, j);
+            if (swap) sb.sd.kind[j] |= S1_SWAP;
         }
         sb.last[t] = j;
         sb.placed.push_back({2 * j, idx});
