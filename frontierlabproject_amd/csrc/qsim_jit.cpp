@@ -44,10 +44,7 @@ bool skip_pass(const Plan &plan, size_t p, int nv) { return nv == 1 && plan.pass
 constexpr int kPairF[6] = {0, 2, 0, 1, 0, 1};
 constexpr int kPairS[6] = {1, 3, 2, 3, 3, 2};
 
-// lut >= 0: the step's base slot is L<lut> (the lane part, read from the
-// kernel's lane table) xor its wave part, instead of being computed per
-// thread (needs one register group per thread: NG == tpb).
-void emit_step(std::ostringstream &o, const StepDev &sd, int E, const char *T, int nv, int tpb, int lut = -1)
+void emit_step(std::ostringstream &o, const StepDev &sd, int E, const char *T, int nv, int tpb)
 {
     const int NG = E >> kG;
     int moff[16];
@@ -80,16 +77,8 @@ void emit_step(std::ostringstream &o, const StepDev &sd, int E, const char *T, i
     // LDS addressing in bytes (slot * amplitude size): the xor with each
     // register offset then needs no shift
     const int esz = std::string(T) == "float" ? 8 : 16;
-    if (lut >= 0 && !loop && !guard) {
-        // lane part from the kernel's table (bytes), the wave part (uniform)
-        // on the scalar unit: swz and the zero insertion are linear
-        o << "    const int sb = (L" << lut << " ^ (swz((int)ins0(ins0(ins0(ins0(wv << 6, " << sd.sbits[0] << "), "
-          << sd.sbits[1] << "), " << sd.sbits[2] << "), " << sd.sbits[3] << ")) * " << esz << "))"
-          << (bit0 ? " & ~" + std::to_string(esz) : std::string()) << ";\n";
-    } else {
-        o << "    const int sb = (swz((int)ins0(ins0(ins0(ins0(g32, " << sd.sbits[0] << "), " << sd.sbits[1] << "), "
-          << sd.sbits[2] << "), " << sd.sbits[3] << "))" << (bit0 ? " & ~1" : "") << ") * " << esz << ";\n";
-    }
+    o << "    const int sb = (swz((int)ins0(ins0(ins0(ins0(g32, " << sd.sbits[0] << "), " << sd.sbits[1] << "), "
+      << sd.sbits[2] << "), " << sd.sbits[3] << "))" << (bit0 ? " & ~1" : "") << ") * " << esz << ";\n";
     for (int m = 0; m < 16; ++m) moff[m] *= esz;
     // vectors: a (psi, in c0) and, with nv == 2, l (lam, in c1)
     const char *vec[2] = {"a", "l"}, *lds[2] = {"c0", "c1"};
@@ -269,10 +258,11 @@ std::string bit_moves(const char *x, const int *dst, int nbits, const char *ty)
 
 // Per-lane constants of a pass kernel, computed here and compiled into the
 // kernel as a table (one 64-entry row per item, read once per thread at the
-// start): the lane part of each copy's offset and LDS slot, and of each
-// register step's base slot.  Offsets, slots and the zero insertion of the
-// steps are linear over GF(2) and the lane and wave parts of the thread index
-// have disjoint bits, so the wave part (uniform) is added on the scalar unit.
+// start): the lane part of each copy's global offset and LDS slot.  Offsets
+// and slots are linear over GF(2) and the lane and wave parts of the thread
+// index have disjoint bits, so the wave part (uniform) is added on the scalar
+// unit.  (Per-step lane parts read the same way were measured slower than
+// computing each step's base per thread: their loads land before the steps.)
 struct LaneTable {
     std::vector<int32_t> v;  // rows of 64
     int add_row(const std::vector<int32_t> &r)
@@ -288,22 +278,6 @@ uint64_t deposit(uint64_t x, const int *to, int nbits)
     for (int k = 0; k < nbits; ++k)
         if ((x >> k) & 1) r |= 1ull << to[k];
     return r;
-}
-
-uint32_t ins0_host(uint32_t x, int p) { return ((x >> p) << (p + 1)) | (x & ((1u << p) - 1u)); }
-
-// Row of the step's lane parts, or -1 (the step computes its base per
-// thread: several groups per thread, or a dense matrix-core step).
-int step_row(LaneTable &lt, const StepDev &sd, int E, const char *T, int nv, int tpb)
-{
-    const bool mfma = sd.dense >= 0 && nv == 1 && std::string(T) == "float";
-    if ((E >> kG) != tpb || mfma) return -1;
-    const int esz = std::string(T) == "float" ? 8 : 16;
-    std::vector<int32_t> r(64);
-    for (uint32_t l = 0; l < 64; ++l)
-        r[l] = esz * swz_host((int)ins0_host(ins0_host(ins0_host(ins0_host(l, sd.sbits[0]), sd.sbits[1]),
-                                                       sd.sbits[2]), sd.sbits[3]));
-    return lt.add_row(r);
 }
 
 // Specialised chunk copies.  The read index r of a chunk element splits into
@@ -377,6 +351,7 @@ void emit_copy(std::ostringstream &o, const CopyPlan &c, const char *tag, const 
         o << "  {\n    v4f b[" << nb << "];\n";
         for (int pass = 0; pass < 2; ++pass)
             for (int q = 0; q < nb; ++q) {
+                const std::string bq = "b[" + std::to_string(q) + "]";
                 const uint64_t rj = (uint64_t)(j0 + q) << c.tb;
                 const uint64_t Cj = deposit(rj, src, K);
                 const int Sj = swz_host((int)deposit(rj, chunk, K)) * esz;
@@ -387,18 +362,18 @@ void emit_copy(std::ostringstream &o, const CopyPlan &c, const char *tag, const 
                 const std::string la = "reinterpret_cast<v4f *>(reinterpret_cast<char *>(" + std::string(lds) + ") + (" +
                                        slot + "))";
                 if (load && pass == 0) {
-                    o << "    b[" << q << "] = __builtin_nontemporal_load(" << a.str() << ");\n";
+                    o << "    " << bq << " = __builtin_nontemporal_load(" << a.str() << ");\n";
                 } else if (load) {
                     if (dbl || ident)
-                        o << "    *" << la << " = b[" << q << "];\n";
+                        o << "    *" << la << " = " << bq << ";\n";
                     else
-                        o << "    { const int sl = " << slot << "; lds_at(" << lds << ", sl) = cx<float>{b[" << q
-                          << "].x, b[" << q << "].y}; lds_at(" << lds << ", sl ^ " << s1 << ") = cx<float>{b[" << q
-                          << "].z, b[" << q << "].w}; }\n";
+                        o << "    { const int sl = " << slot << "; lds_at(" << lds << ", sl) = cx<float>{" << bq
+                          << ".x, " << bq << ".y}; lds_at(" << lds << ", sl ^ " << s1 << ") = cx<float>{" << bq
+                          << ".z, " << bq << ".w}; }\n";
                 } else if (pass == 0) {
-                    o << "    b[" << q << "] = *" << la << ";\n";
+                    o << "    " << bq << " = *" << la << ";\n";
                 } else {
-                    o << "    __builtin_nontemporal_store(b[" << q << "], " << a.str() << ");\n";
+                    o << "    __builtin_nontemporal_store(" << bq << ", " << a.str() << ");\n";
                 }
             }
         o << "  }\n";
@@ -456,20 +431,16 @@ std::string mapped_pass_source(const Plan &plan, bool dbl, const std::string &na
     LaneTable lt;
     const CopyPlan ci = copy_plan(lt, K, dbl, TPB, Pold, J, CFR), co = copy_plan(lt, K, dbl, TPB, pd.pos, ident, CF);
     if (ci.ok && co.ok) {
-        std::vector<int> rows(pd.nsteps);
-        for (int s = 0; s < pd.nsteps; ++s) rows[s] = step_row(lt, plan.steps[pd.step_off + s], E, T, 1, TPB);
         std::ostringstream b;
         emit_lane_prologue(b, name);
-        for (int r : rows)
-            if (r >= 0) b << "  const int L" << r << " = qtab[" << r * 64 << " + lane];\n";
+        emit_copy_thread(b, ci, "i", Pold, J);
+        emit_copy_thread(b, co, "o", pd.pos, ident);
         b << "  const uint64_t c = xcd_slab(blockIdx.x, " << pd.ncomp << ");\n";
         b << "  const uint64_t base = 0ull" << bit_moves("c", pd.comp, pd.ncomp, "uint64_t") << ";\n";
         b << "  const uint64_t base_in = 0ull" << bit_moves("c", comp_old, pd.ncomp, "uint64_t") << ";\n";
-        emit_copy_thread(b, ci, "i", Pold, J);
-        emit_copy_thread(b, co, "o", pd.pos, ident);
         emit_copy(b, ci, "i", "psi", "c0", "base_in", Pold, J, K, true, dbl);
         b << "  __syncthreads();\n";
-        for (int s = 0; s < pd.nsteps; ++s) emit_step(b, plan.steps[pd.step_off + s], E, T, 1, TPB, rows[s]);
+        for (int s = 0; s < pd.nsteps; ++s) emit_step(b, plan.steps[pd.step_off + s], E, T, 1, TPB);
         emit_copy(b, co, "o", "out", "c0", "base", pd.pos, ident, K, false, dbl);
         return lane_table_source(lt, name) + o.str() + b.str() + "}\n\n";
     }
@@ -534,15 +505,11 @@ std::string jit_pass_source(const Plan &plan, size_t p, bool dbl, int nv, const 
     LaneTable lt;
     const CopyPlan cp = copy_plan(lt, K, dbl, TPB, pd.pos, ident, CF);
     if (cp.ok) {
-        std::vector<int> rows(pd.nsteps);
-        for (int s = 0; s < pd.nsteps; ++s) rows[s] = step_row(lt, plan.steps[pd.step_off + s], E, T, nv, TPB);
         std::ostringstream b;
         emit_lane_prologue(b, name);
-        for (int r : rows)
-            if (r >= 0) b << "  const int L" << r << " = qtab[" << r * 64 << " + lane];\n";
+        emit_copy_thread(b, cp, "", pd.pos, ident);
         b << "  const uint64_t c = xcd_slab(blockIdx.x, " << pd.ncomp << ");\n";
         b << "  const uint64_t base = 0ull" << bit_moves("c", pd.comp, pd.ncomp, "uint64_t") << ";\n";
-        emit_copy_thread(b, cp, "", pd.pos, ident);
         if (p == 0 && plan.zero_input && nv == 1) {
             // the input is |0...0>: the chunk holding amplitude 0 starts as e_0, every other as zeros
             b << "  for (int t = threadIdx.x; t < " << E << "; t += " << TPB << ") c0[t] = cx<" << T << ">{0, 0};\n";
@@ -553,7 +520,7 @@ std::string jit_pass_source(const Plan &plan, size_t p, bool dbl, int nv, const 
         }
         if (nv == 2) emit_copy(b, cp, "", "lam", "c1", "base", pd.pos, ident, K, true, dbl);
         b << "  __syncthreads();\n";
-        for (int s = 0; s < pd.nsteps; ++s) emit_step(b, plan.steps[pd.step_off + s], E, T, nv, TPB, rows[s]);
+        for (int s = 0; s < pd.nsteps; ++s) emit_step(b, plan.steps[pd.step_off + s], E, T, nv, TPB);
         emit_copy(b, cp, "", "psi", "c0", "base", pd.pos, ident, K, false, dbl);
         if (nv == 2) {
             emit_copy(b, cp, "", "lam", "c1", "base", pd.pos, ident, K, false, dbl);
