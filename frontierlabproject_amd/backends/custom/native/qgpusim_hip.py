@@ -16,6 +16,8 @@ raises ``ImportError``.
 """
 import ctypes
 import os
+import sys
+import warnings
 
 import numpy as np
 
@@ -98,16 +100,18 @@ def _load():
         raise ImportError(
             "HIP backend library %s is not built: run "
             "`python -m frontierlabproject_amd.build_native` (hipcc, gfx950)" % LIB_PATH)
-    # PyTorch ships its own HIP runtime, and it finds no device when the
-    # system runtime this library links has initialised the GPU first in the
-    # same process (the reverse order works).  Loading torch first keeps
-    # torch usable beside the library (device tensors for the module gate
-    # functions, autograd); QGPUSIM_NO_TORCH=1 skips it.
-    if not os.environ.get("QGPUSIM_NO_TORCH"):
-        try:
-            import torch  # noqa: F401
-        except ImportError:
-            pass
+    # PyTorch-ROCm wheels bundle their own HIP runtime, run-time compiler and
+    # code-object manager under the same sonames as the system ROCm this
+    # library is built against (libamdhip64.so.7, libhiprtc.so.7,
+    # libamd_comgr.so.3), and the first copy loaded serves the whole process.
+    # Loaded first, the library runs on the system stack it was built with
+    # (torch's device runtime then finds no GPU in this process; its CPU
+    # tensors are unaffected).  Loaded after torch, it would mix torch's
+    # runtime with the system compiler.
+    if "torch" in sys.modules:
+        warnings.warn("PyTorch was imported before the HIP backend library: the library will run on "
+                      "torch's bundled HIP runtime instead of the system ROCm it was built against; "
+                      "import the backend first", RuntimeWarning, stacklevel=3)
     lib = ctypes.CDLL(LIB_PATH)
     for name, (args, res) in _SIGNATURES.items():
         fn = getattr(lib, name)

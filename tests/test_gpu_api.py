@@ -1,9 +1,14 @@
 """The reference's single-gate API (bindings.cpp:589-608, 931-1001,
 1026-1033, 1085-1091) on the GPU against the FP64 oracle: the ``_dev``
 methods of both state classes, and the module functions ``apply_1q_gate`` /
-``apply_2q_gate`` on caller-owned torch device tensors, with their error
-messages; and the autograd wrapper when one simulator serves several
-forwards."""
+``apply_2q_gate`` on caller-owned device vectors, with their error messages;
+and the autograd wrapper when one simulator serves several forwards.
+
+The reference passes CuPy arrays.  CuPy is absent here, and torch's device
+runtime cannot share a process with the library (its wheel bundles another
+HIP runtime under the same sonames), so the device arrays are CuPy-like
+objects over memory the library owns: a state's amplitude buffer exposed
+through ``__cuda_array_interface__``."""
 import numpy as np
 import pytest
 
@@ -21,28 +26,44 @@ def _rand_unitary(k, rng):
     return q * (np.diag(r) / np.abs(np.diag(r)))
 
 
+class _DeviceArray:
+    """CuPy-like device array over a state's amplitude buffer: the library's
+    device memory, read back with get()."""
+
+    def __init__(self, hip, data, shape=None):
+        data = np.asarray(data)
+        n = int(np.log2(data.size))
+        assert 1 << n == data.size
+        self.sv = hip.Statevector(n) if data.dtype == np.complex128 else hip.StatevectorF32(n)
+        self.sv.from_numpy(data.reshape(-1))
+        self.shape = tuple(shape or data.shape)
+        self.__cuda_array_interface__ = {"shape": self.shape, "typestr": data.dtype.str, "strides": None,
+                                         "data": (self.sv.device_ptr(), False), "version": 3}
+
+    def get(self):
+        return self.sv.to_numpy().reshape(self.shape)
+
+
 @pytest.mark.parametrize("cls,dtype", [("Statevector", np.complex128), ("StatevectorF32", np.complex64)])
 def test_dev_methods_match_oracle(hip, cls, dtype):
-    """apply_1q_dev / apply_2q_dev with matrices held as torch device tensors,
-    every position and ordered pair of a 6-qubit state, against the oracle."""
-    import torch
+    """apply_1q_dev / apply_2q_dev with matrices held in device memory, every
+    position and ordered pair of a 6-qubit state, against the oracle."""
     n = 6
     rng = np.random.default_rng(11)
     psi0 = C.random_state(n, 31)
     sv = getattr(hip, cls)(n)
     sv.from_numpy(psi0.astype(dtype))
     ref = psi0.copy()
-    tdt = torch.complex128 if dtype == np.complex128 else torch.complex64
     for q in range(n):
         U = _rand_unitary(1, rng)
-        sv.apply_1q_dev(q, torch.tensor(U, dtype=tdt, device="cuda"))
-        ref = dense.apply(ref, U, [q], n)
+        sv.apply_1q_dev(q, _DeviceArray(hip, U.astype(dtype)))
+        ref = dense.apply(ref, U.astype(dtype), [q], n)
     for q0 in range(n):
         for q1 in range(n):
             if q0 != q1:
                 U = _rand_unitary(2, rng)
-                sv.apply_2q_dev(q0, q1, torch.tensor(U, dtype=tdt, device="cuda"))
-                ref = dense.apply(ref, U, [q0, q1], n)
+                sv.apply_2q_dev(q0, q1, _DeviceArray(hip, U.astype(dtype)))
+                ref = dense.apply(ref, U.astype(dtype), [q0, q1], n)
     got = sv.to_numpy()
     assert np.max(np.abs(got - ref)) < TOL[dtype] * 30
     with pytest.raises(RuntimeError, match="Invalid target qubit"):
@@ -57,29 +78,27 @@ def test_dev_methods_match_oracle(hip, cls, dtype):
 
 
 @pytest.mark.parametrize("dtype", [np.complex64, np.complex128])
-def test_module_gate_functions_on_torch_tensors(hip, dtype):
-    """Module apply_1q_gate / apply_2q_gate on a caller-owned torch device
-    vector (its stream, synchronised on return): every position, the pairs in
-    both orders, a 2x2 given as a torch host tensor; then the refusals."""
-    import torch
+def test_module_gate_functions_on_device_vectors(hip, dtype):
+    """Module apply_1q_gate / apply_2q_gate on a caller-owned device vector
+    (synchronised on return): every position, pairs in both orders, a
+    textbook CX; then the refusals, which leave the vector unchanged."""
     n = 7
     rng = np.random.default_rng(5)
-    psi0 = C.random_state(n, 77)
-    tdt = torch.complex128 if dtype == np.complex128 else torch.complex64
-    psi = torch.tensor(psi0.astype(dtype), device="cuda")
-    ref = psi0.copy()
+    psi0 = C.random_state(n, 77).astype(dtype)
+    psi = _DeviceArray(hip, psi0)
+    ref = psi0.astype(np.complex128)
     for q in range(n):
-        U = _rand_unitary(1, rng)
-        hip.apply_1q_gate(psi, torch.tensor(U) if q % 2 else U, q, n)
+        U = _rand_unitary(1, rng).astype(dtype)
+        hip.apply_1q_gate(psi, U, q, n)
         ref = dense.apply(ref, U, [q], n)
     for q0, q1 in [(0, 1), (1, 0), (2, 6), (6, 2), (3, 5)]:
-        U = _rand_unitary(2, rng)
+        U = _rand_unitary(2, rng).astype(dtype)
         hip.apply_2q_gate(psi, U, q0, q1, n)
         ref = dense.apply(ref, U, [q0, q1], n)
-    # a textbook gate too: CX with control q0 = 4 flips target 0 (little-endian)
+    # CX with control q0 = 4 flips target 0 (little-endian operand order)
     hip.apply_2q_gate(psi, G.matrix("cx"), 4, 0, n)
     ref = dense.apply(ref, G.matrix("cx"), [4, 0], n)
-    got = psi.cpu().numpy()
+    got = psi.get()
     assert np.max(np.abs(got - ref)) < TOL[dtype] * 30
     with pytest.raises(RuntimeError, match="q0 and q1 must be different"):
         hip.apply_2q_gate(psi, np.eye(4), 3, 3, n)
@@ -87,15 +106,19 @@ def test_module_gate_functions_on_torch_tensors(hip, dtype):
         hip.apply_1q_gate(psi, np.eye(2), n, n)
     with pytest.raises(RuntimeError, match="2\\^num_qubits elements"):
         hip.apply_1q_gate(psi, np.eye(2), 0, n + 1)
+    half = _DeviceArray(hip, psi0[: 1 << (n - 1)])
     with pytest.raises(RuntimeError, match="2\\^num_qubits elements"):
-        hip.apply_1q_gate(psi[: 1 << (n - 1)], np.eye(2), 0, n)
+        hip.apply_1q_gate(half, np.eye(2), 0, n)
+    strided = _DeviceArray(hip, np.zeros(2 << n, dtype=dtype))
+    strided.__cuda_array_interface__ = dict(strided.__cuda_array_interface__, shape=(1 << n,),
+                                            strides=(2 * np.dtype(dtype).itemsize,))
     with pytest.raises(RuntimeError, match="contiguous"):
-        hip.apply_1q_gate(torch.zeros(2 << n, dtype=tdt, device="cuda")[::2], np.eye(2), 0, n)
-    with pytest.raises(RuntimeError, match="device array"):
-        hip.apply_1q_gate(torch.zeros(1 << n, dtype=tdt), np.eye(2), 0, n)
+        hip.apply_1q_gate(strided, np.eye(2), 0, n)
+    with pytest.raises(RuntimeError, match="__cuda_array_interface__"):
+        hip.apply_1q_gate(psi0, np.eye(2), 0, n)
     with pytest.raises(RuntimeError, match="U must be a 4x4 matrix"):
         hip.apply_2q_gate(psi, np.eye(2), 0, 1, n)
-    assert np.array_equal(psi.cpu().numpy(), got)
+    assert np.array_equal(psi.get(), got)
 
 
 def test_adjoint_autograd_with_shared_simulator():
