@@ -12,8 +12,9 @@ on the pairs of a random permutation with p = 0.5), complex64 (64 GiB state),
 scheduled by lazy qubit reordering with nL = 18 and run as graphed LDS chunk
 passes.  Permutations move no data by themselves: each tile's first chunk
 pass moves the qubits that tile needs into the low bits (read in the old
-stored order, written in the new one), and the last layout is restored in one
-permutation pass when the state is read.  The timed region is reset (|0...0>,
+stored order, written in the new one), and the last tile's last chunk pass
+writes the state back in logical order (the final layout change handed to
+that tile's call).  The timed region is reset (|0...0>,
 generated by the first chunk pass) + all permutations + all tiles; the state
 stays in HBM throughout.
 
@@ -118,6 +119,14 @@ def build_program_for(hip, qc, nL, dtype=None):
         if ops:
             arr, arena = hip.pack_ops(ops, n, dtype)
             prog.append(("tile", arr, len(ops), arena))
+    # the layout changes after the last tile (to the final layout) go with
+    # it as one call: its last chunk pass writes them
+    k = len(prog)
+    while k > 0 and prog[k - 1][0] == "perm":
+        k -= 1
+    if 0 < k < len(prog) and prog[k - 1][0] == "tile":
+        _, arr, nops, arena = prog[k - 1]
+        prog[k - 1:] = [("tile_perm", arr, nops, arena, prog[k][1], prog[-1][2], len(prog) - k)]
     # program-order op list for the plain path
     plain = []
     for ci in qc.data:
@@ -148,6 +157,8 @@ def run_program(hip, sv, prog, mode, reset=True):
     for item in prog:
         if item[0] == "perm":
             sv.permute(item[1], item[2])
+        elif item[0] == "tile_perm":
+            sv.apply_packed_permute(mode, item[1], item[2], item[3], item[4], item[5])
         else:
             sv.apply_packed(mode, item[1], item[2], item[3])
     sv.synchronize()
@@ -212,7 +223,7 @@ def main():
     bytes_per_pass = 2 * (1 << n) * (16 if c128 else 8)
     log(f"building program: random circuit n={n} depth={args.depth} nL={args.nL} {args.dtype}")
     qc, plan, prog, plain, ngates, t_tr = build_program(hip, n, args.depth, args.nL, args.seed, cdt)
-    nperm = sum(1 for p in prog if p[0] == "perm")
+    nperm = sum(1 if p[0] == "perm" else p[6] if p[0] == "tile_perm" else 0 for p in prog)
     log(f"{len(plan)} tiles, {nperm} permutations, {ngates} gates, transpile {t_tr:.2f} s")
 
     sv = hip.Statevector(n) if c128 else hip.StatevectorF32(n)
@@ -354,8 +365,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"random circuit {n}q depth {args.depth} {args.dtype}, LQR nL={args.nL}, "
                                "graphed LDS chunk passes; each layout change folded into the next tile's first "
-                               "pass, the last one restored by one permutation pass; |0...0> generated by the "
-                               "first pass",
+                               "pass, the final one written by the last tile's last pass; |0...0> generated "
+                               "by the first pass",
                    "n_qubits": n, "depth": args.depth, "nL": args.nL, "seed": args.seed,
                    "tiles": len(plan), "permutations": nperm, "gates": ngates,
                    "folded_permutations_per_circuit": folded,
@@ -428,7 +439,7 @@ def main_qft(args, hip, np):
     qc = W.qft_circuit(n)
     qc.save_statevector()  # final all-qubit tile: output in standard qubit order
     _, plan, prog, _, ngates, t_tr = build_program_for(hip, qc, nL)
-    nperm = sum(1 for p in prog if p[0] == "perm")
+    nperm = sum(1 if p[0] == "perm" else p[6] if p[0] == "tile_perm" else 0 for p in prog)
     log(f"qft {n}q: {len(plan)} tiles, {nperm} permutations, {ngates} gates")
     psi0 = W.random_state(n, seed)
     sv = hip.StatevectorF32(n)

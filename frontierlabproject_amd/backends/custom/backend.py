@@ -19,8 +19,11 @@ Differences from the reference, each deliberate:
   without capturing a graph, and the metrics report "cooperative";
 * the state is returned in the circuit's qubit order also when the plan has
   no final all-qubit tile (the reference then returns it in the last tile's
-  layout, passes.py:213-227): one lazy permute back to the identity layout,
-  folded into the read's permutation pass;
+  layout, passes.py:213-227);
+* the layout changes after the last tile with ops (the final all-qubit
+  tile's, and the return to the circuit's order) are written by that tile's
+  last chunk pass when it has room for them (``apply_tile_permute``), else
+  recorded and applied by one permutation pass when the state is read;
 * there is no CuPy fallback simulator: the HIP library is required.
 
 ``tile_mode``:
@@ -38,6 +41,9 @@ import numpy as np
 from ...circuit import DECOMPOSITIONS, decompose
 from ...transpiler.passes import Tile
 from .native import qgpusim_hip as qgpusim_cuda
+from .native.qgpusim_hip import COOPERATIVE, GRAPHED, SEQUENTIAL
+
+_TILE_MODE_CODES = {"cooperative": COOPERATIVE, "graphed": GRAPHED, "sequential": SEQUENTIAL}
 
 USE_NATIVE = True
 
@@ -218,37 +224,52 @@ def run_custom_backend(qc, tile_plan: List[Tile], task: str, shots: int = None,
 
     phase = _global_phase(qc)  # applied with the first op (or to the result when nothing runs)
     sv = qgpusim_cuda.StatevectorF32(num_qubits) if use_float32 else qgpusim_cuda.Statevector(num_qubits)
-    layout = list(range(num_qubits))
-    for tile in tile_plan:
+    identity = list(range(num_qubits))
+    t0 = time.perf_counter()
+    tile_ops_all = [build_tile_ops(tile, qubit_map, np_dtype) for tile in tile_plan]
+    t_apply += time.perf_counter() - t0
+    # The layout changes after the last tile with ops, up to the circuit's
+    # qubit order, are written by that tile's last chunk pass (one call; the
+    # same result as applying the ops, then permuting).
+    last_ops = max([i for i, (ops_i, _) in enumerate(tile_ops_all) if ops_i], default=-1)
+    mode_code = _TILE_MODE_CODES[tile_mode]
+    layout = list(identity)
+    for i, tile in enumerate(tile_plan):
         if tile.layout != layout:
             t0 = time.perf_counter()
-            sv.permute(layout, tile.layout)
+            if i <= last_ops:
+                sv.permute(layout, tile.layout)
             t_perm += time.perf_counter() - t0
             num_reorders += 1
             layout = tile.layout
         t0 = time.perf_counter()
-        tile_ops, ng = build_tile_ops(tile, qubit_map, np_dtype)
+        tile_ops, ng = tile_ops_all[i]
         num_gates += ng
         if tile_ops:
             if enable_fusion:
                 tile_ops = fuse_tile_ops(tile_ops, np_dtype)
             tile_ops, phase = fold_global_phase(tile_ops, phase, np_dtype), 0.0
             num_fused_ops += len(tile_ops)
-            if tile_mode == "cooperative":
+            if i == last_ops and layout != identity:
+                sv.apply_tile_permute(mode_code, tile_ops, layout, identity)
+            elif tile_mode == "cooperative":
                 sv.apply_tile_cooperative(tile_ops)
-                num_kernel_launches += 1
             elif tile_mode == "graphed":
                 sv.apply_tile_graphed(tile_ops)
-                num_graph_replays += 1
             else:
                 sv.apply_tile_dev(tile_ops)
+            if tile_mode == "cooperative":
+                num_kernel_launches += 1
+            elif tile_mode == "graphed":
+                num_graph_replays += 1
+            else:
                 num_kernel_launches += len(tile_ops)
         t_apply += time.perf_counter() - t0
 
-    identity = list(range(num_qubits))
     if layout != identity:  # no final all-qubit tile: back to the circuit's qubit order
         t0 = time.perf_counter()
-        sv.permute(layout, identity)
+        if last_ops < 0:
+            sv.permute(layout, identity)
         t_perm += time.perf_counter() - t0
         layout = identity
     t_sync0 = time.perf_counter()

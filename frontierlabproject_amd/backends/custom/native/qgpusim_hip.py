@@ -63,6 +63,7 @@ _SIGNATURES = {
     "qsim_permute": ([_P, _P, _P, _I], _I),
     "qsim_validate_ops": ([_I, _P, _I, _U64], _I),
     "qsim_apply_ops": ([_P, _I, _P, _I, _P, _U64], _I),
+    "qsim_apply_ops_permute": ([_P, _I, _P, _I, _P, _U64, _P, _P, _I], _I),
     "qsim_apply_1q": ([_P, _I, _P], _I),
     "qsim_apply_2q": ([_P, _I, _I, _P], _I),
     "qsim_apply_diagonal_1q": ([_P, _I, _D, _D, _D, _D], _I),
@@ -77,7 +78,7 @@ _SIGNATURES = {
     "qsim_get_counters": ([_P, _P, _I], _I),
     "qsim_profile": ([_P, _I], _I),
     "qsim_profile_read": ([_P, _P, _P, _I], _I),
-    "qsim_plan_describe": ([_I, _I, _P, _I, _P, _U64, _I, _I, _P, _I, _P], _I),
+    "qsim_plan_describe": ([_I, _I, _P, _I, _P, _U64, _I, _I, _P, _P, _P, _I, _P], _I),
     "qsim_set_jit": ([_P, _I], _I),
     "qsim_batch_create": ([ctypes.POINTER(_P)], _I),
     "qsim_batch_destroy": ([_P], _I),
@@ -85,7 +86,7 @@ _SIGNATURES = {
     "qsim_batch_load": ([_P, _I, _P, _P, _P, _P, _P, _P, _P, _U64, _P], _I),
     "qsim_batch_run": ([_P, _I, _P, _P, _P, _P], _I),
     "qsim_batch_info": ([_P, _P, _I], _I),
-    "qsim_jit_compile_check": ([_I, _I, _P, _I, _P, _U64, _I, _I, _I, _P], _I),
+    "qsim_jit_compile_check": ([_I, _I, _P, _I, _P, _U64, _I, _I, _I, _P, _P, _P], _I),
     "qsim_choose_layout": ([_I, _I, _I, _P, _P, _I, _P], _I),
     "qsim_set_fusion": ([_P, _I], _I),
     "qsim_device_memory": ([_P, _P], _I),
@@ -299,6 +300,24 @@ class _StatevectorBase:
         _check(lib.qsim_apply_ops(self._h, mode, ctypes.addressof(arr), nops,
                                   arena.ctypes.data, arena.size))
 
+    def apply_packed_permute(self, mode, arr, nops, arena, layout_old, layout_new):
+        """apply_packed then permute(layout_old, layout_new) as one call: in
+        the chunked modes the op list's last chunk pass writes the new layout
+        (no separate permutation pass) when it has room for the output's row
+        bits.  Same results and errors as the two calls."""
+        _check_packed(arr, nops, arena, self._DTYPE)
+        lo, ln = list(layout_old), list(layout_new)
+        a = np.ascontiguousarray(lo, dtype=np.int32)
+        b = np.ascontiguousarray(ln, dtype=np.int32)
+        _check(lib.qsim_apply_ops_permute(self._h, mode, ctypes.addressof(arr), nops, arena.ctypes.data, arena.size,
+                                          a.ctypes.data, b.ctypes.data, len(lo) if len(lo) == len(ln) else -1))
+
+    def apply_tile_permute(self, mode, ops, layout_old, layout_new):
+        """apply_tile_{dev,cooperative,graphed} (mode) then permute, as one
+        call (see apply_packed_permute)."""
+        arr, arena = pack_ops(ops, self._n, self._DTYPE)
+        self.apply_packed_permute(mode, arr, len(ops), arena, layout_old, layout_new)
+
     # -- single gates (bindings.cpp:499-544, 589-619) --
     def apply_1q(self, target_qubit, U):
         m = _matrix(U, 1, self._DTYPE)
@@ -508,21 +527,31 @@ def device_memory():
     return f.value, t.value
 
 
-def jit_compile_check(num_qubits, arr, nops, arena, K, cmin, fuse=False, in_map=None):
+def jit_compile_check(num_qubits, arr, nops, arena, K, cmin, fuse=False, in_map=None, out_map=None,
+                      folded=None):
     """Host-only: plan packed ops (after a pending layout change in_map, bit p
-    -> in_map[p]) and compile every specialised pass kernel for gfx950 without
-    loading it; returns the number of passes.  fuse: a fusion mode
-    (Statevector.set_fusion) or a bool for mode 1."""
+    -> in_map[p]; followed by a layout change out_map, bit p -> out_map[p])
+    and compile every specialised pass kernel for gfx950 without loading it;
+    returns the number of passes.  fuse: a fusion mode
+    (Statevector.set_fusion) or a bool for mode 1.  folded: a list that
+    receives whether the last pass writes the out_map order."""
     _check_packed(arr, nops, arena)
     code = C128 if arena.dtype == np.complex128 else C64
-    m = None if in_map is None else np.ascontiguousarray(in_map, dtype=np.int32)
-    if m is not None and m.size != num_qubits:
-        raise ValueError("in_map must have num_qubits entries")
+    maps = []
+    for mp in (in_map, out_map):
+        m = None if mp is None else np.ascontiguousarray(mp, dtype=np.int32)
+        if m is not None and m.size != num_qubits:
+            raise ValueError("in_map/out_map must have num_qubits entries")
+        maps.append(m)
+    flag = ctypes.c_int(0)
     npass = lib.qsim_jit_compile_check(num_qubits, code, ctypes.addressof(arr), nops, arena.ctypes.data,
                                        arena.size, K, cmin, int(fuse),
-                                       None if m is None else m.ctypes.data)
+                                       None if maps[0] is None else maps[0].ctypes.data,
+                                       None if maps[1] is None else maps[1].ctypes.data, ctypes.byref(flag))
     if npass < 0:
         raise RuntimeError(lib.qsim_last_error().decode())
+    if folded is not None:
+        folded.append(bool(flag.value))
     return npass
 
 
@@ -541,16 +570,25 @@ def choose_layout(num_qubits, perm_map, arr, nops, dtype=np.complex64, cmin=None
     return out.tolist()
 
 
-def plan_describe(num_qubits, arr, nops, arena, K, cmin, steps=False):
+def plan_describe(num_qubits, arr, nops, arena, K, cmin, steps=False, in_map=None, out_map=None):
     """Host-only chunk plan: ([(K, cfast, nsteps, pos[, steps])], execution order),
-    a step being (register bits, rounds, dense)."""
+    a step being (register bits, rounds, dense); in_map / out_map as for
+    jit_compile_check."""
     _check_packed(arr, nops, arena)
     code = C128 if arena.dtype == np.complex128 else C64
     cap = 4096 + 128 * max(1, nops)
     desc = np.zeros(cap, dtype=np.int32)
     order = np.zeros(max(1, nops), dtype=np.int32)
+    maps = []
+    for mp in (in_map, out_map):
+        m = None if mp is None else np.ascontiguousarray(mp, dtype=np.int32)
+        if m is not None and m.size != num_qubits:
+            raise ValueError("in_map/out_map must have num_qubits entries")
+        maps.append(m)
     npass = lib.qsim_plan_describe(num_qubits, code, ctypes.addressof(arr), nops, arena.ctypes.data,
-                                   arena.size, K, cmin, desc.ctypes.data, cap, order.ctypes.data)
+                                   arena.size, K, cmin, None if maps[0] is None else maps[0].ctypes.data,
+                                   None if maps[1] is None else maps[1].ctypes.data, desc.ctypes.data, cap,
+                                   order.ctypes.data)
     if npass < 0:
         raise RuntimeError(lib.qsim_last_error().decode())
     passes = []

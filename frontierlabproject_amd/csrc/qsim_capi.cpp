@@ -103,7 +103,9 @@ struct GraphEntry {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     void *desc = nullptr;
-    bool swaps = false;  // its first pass writes the other buffer (folded layout change)
+    int swaps = 0;  // passes writing the other buffer (folded layout changes): the buffers swap this often
+    int folds = 0;  // layout changes those passes carry (a single pass can carry two)
+    bool post_done = false;      // its last pass writes the layout change that follows the list
     bool after_pending = false;  // stored layout after the replay (perm_map)
     int32_t after_map[64];
 };
@@ -344,17 +346,20 @@ int enqueue_passes(qsim_state *st, const Plan &plan, const DescImage &img, char 
         const PassDev &hp = plan.passes[p];
         const int nblocks = 1 << (st->n - hp.K);
         ProfScope ps(st, 0);
-        if (p == 0 && !plan.in_map.empty()) {
-            // folded layout change: read psi in the old layout, write tmp in
-            // the new one; later passes work in place on the new buffer
+        const bool remap = (p == 0 && !plan.in_map.empty()) || (p + 1 == plan.passes.size() && !plan.out_map.empty());
+        if (remap) {
+            // folded layout change: read psi in the old layout (first pass)
+            // and/or write the output order (last pass) into tmp; the other
+            // passes work in place
+            if (!jit || p >= jit->size() || !(*jit)[p]) return fail("internal: folded layout change without its kernel");
             void *in = st->psi, *out = st->tmp;
             const void *mats = dm;
             void *args[] = {&in, &out, &mats};
-            HIP_TRY(hipModuleLaunchKernel((*jit)[0], (unsigned)nblocks, 1, 1, (unsigned)jit_block_threads(hp.K, 1),
+            HIP_TRY(hipModuleLaunchKernel((*jit)[p], (unsigned)nblocks, 1, 1, (unsigned)jit_block_threads(hp.K, 1),
                                           1, 1, 0, st->stream, args, nullptr));
             std::swap(st->psi, st->tmp);
             st->counters[5]++;
-            st->counters[6]++;
+            st->counters[6] += (p == 0 && !plan.in_map.empty()) + (p + 1 == plan.passes.size() && !plan.out_map.empty());
         } else if (jit && p < jit->size() && (*jit)[p]) {
             void *psi = st->psi;
             void *lamp = lam;
@@ -381,7 +386,7 @@ int enqueue_passes(qsim_state *st, const Plan &plan, const DescImage &img, char 
 }
 
 int plan_for(qsim_state *st, const qsim_op *ops, int nops, const void *mats, int max_probes, Plan &plan,
-             const int32_t *in_map = nullptr)
+             const int32_t *in_map = nullptr, const int32_t *out_map = nullptr)
 {
     std::vector<PlanOp> pops = to_plan_ops(ops, nops);
     std::string err;
@@ -389,7 +394,7 @@ int plan_for(qsim_state *st, const qsim_op *ops, int nops, const void *mats, int
     // amplitudes each
     const int K = max_probes > 1 ? std::min(st->K, st->dtype == QSIM_C64 ? 12 : 11) : st->K;
     if (!plan_passes(st->n, pops.data(), nops, mats, (int)amp_bytes(st->dtype), K, st->cmin, max_probes, plan,
-                     err, max_probes == 1 ? st->fuse : 0, in_map))
+                     err, max_probes == 1 ? st->fuse : 0, in_map, out_map))
         return fail(err);
     return 0;
 }
@@ -482,23 +487,49 @@ void choose_layout(const qsim_state *st, const qsim_op *ops, int nops, int32_t *
 // differs from the logical order.
 struct LayoutAfter {
     bool pending = false;
+    bool post_done = false;  // a layout change after the list was written by its last pass
     int32_t map[64];
 };
 
+// out_map of a layout change `post` (pmap form: position p moves to
+// post[p]) applied after an op list that leaves stored bit p holding logical
+// bit at[p] (identity when null); false when it moves nothing.
+bool post_out_map(int n, const int32_t *at, const int32_t *post, int32_t *out)
+{
+    bool move = false;
+    for (int p = 0; p < n; ++p) {
+        out[p] = post[at ? at[p] : p];
+        move |= out[p] != p;
+    }
+    return move;
+}
+
+// With `post` (a layout change applied after the list, pmap form), the last
+// pass writes the state in the order that change leaves (when it has room
+// for the output's row bits and specialised kernels apply): after.post_done
+// then says so, and after describes the stored order after both.
 template <typename R>
 int plan_and_compile(qsim_state *st, const qsim_op *ops, int nops, const void *mats, Plan &plan,
-                     std::vector<hipFunction_t> &fns, LayoutAfter &after)
+                     std::vector<hipFunction_t> &fns, LayoutAfter &after, const int32_t *post = nullptr)
 {
     const bool keep = can_fold(st);
     if (st->perm_pending && !keep && flush_perm(st)) return -1;
     after.pending = false;
+    after.post_done = false;
+    int32_t omap[64];
     if (!keep) {
-        if (plan_for(st, ops, nops, mats, 1, plan)) return -1;
+        const bool fold_out = post && st->jit && post_out_map(st->n, nullptr, post, omap);
+        if (plan_for(st, ops, nops, mats, 1, plan, nullptr, fold_out ? omap : nullptr)) return -1;
         // a pending |0...0> is generated by a specialised first pass; forward
         // lists carry no probes, so with specialised kernels on, every pass has
         // one.  A graph captured from this plan is keyed on the pending |0...0>,
         // so it must never rely on an eager write made outside the capture.
         plan.zero_input = st->zero_pending && st->jit;
+        if (!plan.out_map.empty() && !jit_mapped_last(plan, sizeof(R) == 8, 1)) plan.out_map.clear();
+        if (!plan.out_map.empty()) {
+            after.post_done = true;  // the state leaves in the logical order after `post`
+            if (ensure_tmp(st)) return -1;
+        }
         if (jit_for(st, plan, sizeof(R) == 8, 1, fns)) return -1;
         if (plan.zero_input && (fns.empty() || !fns[0]))
             return fail("internal: generated first pass without its kernel");
@@ -520,13 +551,22 @@ int plan_and_compile(qsim_state *st, const qsim_op *ops, int nops, const void *m
         after.map[tgt[p]] = p;
     }
     for (int p = 0; p < n; ++p) after.pending |= after.map[p] != p;
-    if (plan_for(st, mapped.data(), nops, mats, 1, plan, move ? in_map : nullptr)) return -1;
+    const bool fold_out = post && post_out_map(n, after.map, post, omap);
+    if (plan_for(st, mapped.data(), nops, mats, 1, plan, move ? in_map : nullptr, fold_out ? omap : nullptr))
+        return -1;
+    if (!plan.out_map.empty() && !jit_mapped_last(plan, sizeof(R) == 8, 1)) plan.out_map.clear();
+    if (!plan.out_map.empty()) {
+        after.post_done = true;  // the state leaves in the logical order after `post`
+        after.pending = false;
+        for (int p = 0; p < n; ++p) after.map[p] = p;
+    }
     if (jit_for(st, plan, sizeof(R) == 8, 1, fns)) return -1;
     if (move) {
         if (!jit_mapped_first(plan, sizeof(R) == 8, 1) || fns.empty() || !fns[0])
             return fail("internal: folded layout change without its kernel");
-        if (ensure_tmp(st)) return -1;
     }
+    if (move || !plan.out_map.empty())
+        if (ensure_tmp(st)) return -1;
     return 0;
 }
 
@@ -536,14 +576,30 @@ void set_layout(qsim_state *st, const LayoutAfter &after)
     std::memcpy(st->perm_map, after.map, sizeof(int32_t) * st->n);
 }
 
+// Records a layout change (pmap form: position p moves to map[p]) in the
+// stored bit order, as qsim_permute does: no data moves.
+void compose_layout(qsim_state *st, const int32_t *map)
+{
+    if (st->zero_pending) return;  // a bit permutation leaves |0...0> as it is
+    int32_t comp[64];
+    for (int p = 0; p < st->n; ++p) comp[p] = st->perm_pending ? map[st->perm_map[p]] : map[p];
+    bool ident = true;
+    for (int p = 0; p < st->n; ++p) ident &= comp[p] == p;
+    st->perm_pending = !ident;
+    std::memcpy(st->perm_map, comp, sizeof(int32_t) * st->n);
+}
+
+// `post` (or null): a layout change applied after the list (pmap form),
+// written by the last pass when it can be, else recorded as by qsim_permute.
 template <typename R>
-int run_chunked(qsim_state *st, const qsim_op *ops, int nops, const void *mats, uint64_t mat_count)
+int run_chunked(qsim_state *st, const qsim_op *ops, int nops, const void *mats, uint64_t mat_count,
+                const int32_t *post = nullptr)
 {
     (void)mat_count;
     Plan plan;
     std::vector<hipFunction_t> fns;
     LayoutAfter after;
-    if (plan_and_compile<R>(st, ops, nops, mats, plan, fns, after)) return -1;
+    if (plan_and_compile<R>(st, ops, nops, mats, plan, fns, after, post)) return -1;
     DescImage img;
     img.build<R>(plan);
     void *dev = nullptr;
@@ -551,11 +607,13 @@ int run_chunked(qsim_state *st, const qsim_op *ops, int nops, const void *mats, 
     if (enqueue_passes<R>(st, plan, img, static_cast<char *>(dev), nullptr, nullptr, &fns)) return -1;
     set_layout(st, after);
     st->zero_pending = false;
+    if (post && !after.post_done) compose_layout(st, post);
     return 0;
 }
 
 template <typename R>
-int run_graphed(qsim_state *st, const qsim_op *ops, int nops, const void *mats, uint64_t mat_count)
+int run_graphed(qsim_state *st, const qsim_op *ops, int nops, const void *mats, uint64_t mat_count,
+                const int32_t *post = nullptr)
 {
     // a layout change that cannot fold is applied first, so the key sees
     // the buffer the passes will use; so is a pending |0...0> that no
@@ -579,31 +637,36 @@ int run_graphed(qsim_state *st, const qsim_op *ops, int nops, const void *mats, 
     put(&st->zero_pending, sizeof(bool));
     put(ops, sizeof(qsim_op) * nops);
     put(mats, sizeof(cx<R>) * mat_count);
+    const bool has_post = post != nullptr;
+    put(&has_post, sizeof(bool));
+    if (post) put(post, sizeof(int32_t) * st->n);
     for (auto it = st->graphs.begin(); it != st->graphs.end(); ++it) {
         if (it->key == key) {
             st->graphs.splice(st->graphs.begin(), st->graphs, it);
             HIP_TRY(hipGraphLaunch(st->graphs.front().exec, st->stream));
             const GraphEntry &g = st->graphs.front();
-            if (g.swaps) {  // replayed a folded layout change
-                std::swap(st->psi, st->tmp);
-                st->counters[6]++;
-            }
+            if (g.swaps & 1) std::swap(st->psi, st->tmp);  // replayed folded layout changes
+            st->counters[6] += g.folds;
             st->perm_pending = g.after_pending;
             std::memcpy(st->perm_map, g.after_map, sizeof(int32_t) * st->n);
             st->zero_pending = false;
             st->counters[1]++;
+            if (post && !g.post_done) compose_layout(st, post);
             return 0;
         }
     }
     Plan plan;
     std::vector<hipFunction_t> fns;
     LayoutAfter after;
-    if (plan_and_compile<R>(st, ops, nops, mats, plan, fns, after)) return -1;
+    if (plan_and_compile<R>(st, ops, nops, mats, plan, fns, after, post)) return -1;
     DescImage img;
     img.build<R>(plan);
     GraphEntry e;
     e.key.swap(key);
-    e.swaps = !plan.in_map.empty();
+    e.folds = (int)!plan.in_map.empty() + (int)!plan.out_map.empty();
+    e.swaps = (int)!plan.in_map.empty() + (int)!plan.out_map.empty() -
+              (int)(plan.passes.size() == 1 && !plan.in_map.empty() && !plan.out_map.empty());
+    e.post_done = after.post_done;
     e.after_pending = after.pending;
     std::memcpy(e.after_map, after.map, sizeof(int32_t) * st->n);
     HIP_TRY(hipMalloc(&e.desc, img.bytes.size()));
@@ -657,6 +720,7 @@ int run_graphed(qsim_state *st, const qsim_op *ops, int nops, const void *mats, 
     st->counters[1]++;
     set_layout(st, after);
     st->zero_pending = false;
+    if (post && !after.post_done) compose_layout(st, post);
     while (st->graphs.size() > kMaxGraphs) {
         HIP_TRY(hipStreamSynchronize(st->stream));
         GraphEntry &old = st->graphs.back();
@@ -672,16 +736,22 @@ int run_graphed(qsim_state *st, const qsim_op *ops, int nops, const void *mats, 
 bool chunk_ok(const qsim_state *st) { return st->n >= 6; }
 
 template <typename R>
-int apply_ops_t(qsim_state *st, int mode, const qsim_op *ops, int nops, const void *mats, uint64_t mat_count)
+int apply_ops_t(qsim_state *st, int mode, const qsim_op *ops, int nops, const void *mats, uint64_t mat_count,
+                const int32_t *post = nullptr)
 {
-    if (nops == 0) return 0;
-    if (mode == QSIM_SEQUENTIAL || !chunk_ok(st)) {
-        if (flush_perm(st)) return -1;
-        return run_sequential<R>(st, ops, nops, mats);
+    if (nops > 0 && mode != QSIM_SEQUENTIAL && chunk_ok(st)) {
+        if (mode == QSIM_COOPERATIVE) return run_chunked<R>(st, ops, nops, mats, mat_count, post);
+        if (mode == QSIM_GRAPHED) return run_graphed<R>(st, ops, nops, mats, mat_count, post);
+        return fail("apply_tile: unknown tile mode " + std::to_string(mode));
     }
-    if (mode == QSIM_COOPERATIVE) return run_chunked<R>(st, ops, nops, mats, mat_count);
-    if (mode == QSIM_GRAPHED) return run_graphed<R>(st, ops, nops, mats, mat_count);
-    return fail("apply_tile: unknown tile mode " + std::to_string(mode));
+    if (nops > 0) {
+        if (mode != QSIM_SEQUENTIAL && mode != QSIM_COOPERATIVE && mode != QSIM_GRAPHED)
+            return fail("apply_tile: unknown tile mode " + std::to_string(mode));
+        if (flush_perm(st)) return -1;
+        if (run_sequential<R>(st, ops, nops, mats)) return -1;
+    }
+    if (post) compose_layout(st, post);
+    return 0;
 }
 
 int build_perm_args(int n, const int32_t *map, int smax, PermArgs &P)
@@ -1032,6 +1102,26 @@ int qsim_apply_ops(qsim_state *st, int mode, const qsim_op *ops, int nops, const
     return apply_ops_t<double>(st, mode, ops, nops, mats, mat_count);
 }
 
+int qsim_apply_ops_permute(qsim_state *st, int mode, const qsim_op *ops, int nops, const void *mats,
+                           uint64_t mat_count, const int32_t *layout_old, const int32_t *layout_new, int num_qubits)
+{
+    if (!st) return fail("null state");
+    if (check_ops(st->n, ops, nops, mat_count, false)) return -1;
+    int32_t map[64];
+    if (num_qubits != st->n || !layout_old || !layout_new) {
+        // as apply then permute: the op list runs, the layout change is refused
+        if (qsim_apply_ops(st, mode, ops, nops, mats, mat_count)) return -1;
+        return fail("layout_old/layout_new must have length num_qubits");
+    }
+    if (permute_map_impl(st->n, layout_old, layout_new, map)) {
+        const std::string msg = g_err;
+        if (qsim_apply_ops(st, mode, ops, nops, mats, mat_count)) return -1;
+        return fail(msg);
+    }
+    if (st->dtype == QSIM_C64) return apply_ops_t<float>(st, mode, ops, nops, mats, mat_count, map);
+    return apply_ops_t<double>(st, mode, ops, nops, mats, mat_count, map);
+}
+
 int qsim_apply_1q(qsim_state *st, int target_qubit, const void *U)
 {
     if (!st) return fail("null state");
@@ -1214,8 +1304,10 @@ int qsim_set_fusion(qsim_state *st, int mode)
 }
 
 int qsim_jit_compile_check(int num_qubits, int dtype, const qsim_op *ops, int nops, const void *mats,
-                           uint64_t mat_count, int K, int cmin, int fuse, const int32_t *in_map)
+                           uint64_t mat_count, int K, int cmin, int fuse, const int32_t *in_map,
+                           const int32_t *out_map, int *out_folded)
 {
+    if (out_folded) *out_folded = 0;
     if (dtype != QSIM_C64 && dtype != QSIM_C128) return fail("unknown dtype");
     if (check_ops(num_qubits, ops, nops, mat_count, true)) return -1;
     std::vector<PlanOp> pops = to_plan_ops(ops, nops);
@@ -1226,8 +1318,10 @@ int qsim_jit_compile_check(int num_qubits, int dtype, const qsim_op *ops, int no
     std::string err;
     if (!plan_passes(num_qubits, pops.data(), nops, mats, (int)amp_bytes(dtype), K, cmin,
                      nv == 2 ? kMaxProbes : 1, plan, err, nv == 1 ? fuse : 0,
-                     nv == 1 ? in_map : nullptr))
+                     nv == 1 ? in_map : nullptr, nv == 1 ? out_map : nullptr))
         return fail(err);
+    if (!plan.out_map.empty() && !jit_mapped_last(plan, dtype == QSIM_C128, nv)) plan.out_map.clear();
+    if (out_folded) *out_folded = plan.out_map.empty() ? 0 : 1;
     if (!jit_compile_only(plan, dtype == QSIM_C128, nv, "gfx950", err))
         return fail("run-time kernel compile: " + err);
     return (int)plan.passes.size();
@@ -1373,14 +1467,17 @@ int qsim_get_counters(const qsim_state *st, int64_t *out, int n)
 }
 
 int qsim_plan_describe(int num_qubits, int dtype, const qsim_op *ops, int nops, const void *mats,
-                       uint64_t mat_count, int K, int cmin, int32_t *desc, int desc_cap, int32_t *order)
+                       uint64_t mat_count, int K, int cmin, const int32_t *in_map, const int32_t *out_map,
+                       int32_t *desc, int desc_cap, int32_t *order)
 {
     if (dtype != QSIM_C64 && dtype != QSIM_C128) return fail("unknown dtype");
     if (check_ops(num_qubits, ops, nops, mat_count, true)) return -1;
     std::vector<PlanOp> pops = to_plan_ops(ops, nops);
     Plan plan;
     std::string err;
-    if (!plan_passes(num_qubits, pops.data(), nops, mats, (int)amp_bytes(dtype), K, cmin, kMaxProbes, plan, err))
+    const bool maps = in_map || out_map;
+    if (!plan_passes(num_qubits, pops.data(), nops, mats, (int)amp_bytes(dtype), K, cmin, maps ? 1 : kMaxProbes,
+                     plan, err, 0, in_map, out_map))
         return fail(err);
     int w = 0;
     for (const PassDev &p : plan.passes) {

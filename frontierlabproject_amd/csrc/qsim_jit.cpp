@@ -371,7 +371,12 @@ void emit_copy(std::ostringstream &o, const CopyPlan &c, const char *tag, const 
                           << ".x, " << bq << ".y}; lds_at(" << lds << ", sl ^ " << s1 << ") = cx<float>{" << bq
                           << ".z, " << bq << ".w}; }\n";
                 } else if (pass == 0) {
-                    o << "    " << bq << " = *" << la << ";\n";
+                    if (dbl || ident)
+                        o << "    " << bq << " = *" << la << ";\n";
+                    else
+                        o << "    { const int sl = " << slot << "; const cx<float> x0 = lds_at(" << lds
+                          << ", sl), x1 = lds_at(" << lds << ", sl ^ " << s1 << "); " << bq
+                          << " = v4f{x0.re, x0.im, x1.re, x1.im}; }\n";
                 } else {
                     o << "    __builtin_nontemporal_store(" << bq << ", " << a.str() << ");\n";
                 }
@@ -401,8 +406,64 @@ void emit_lane_prologue(std::ostringstream &o, const std::string &name)
 
 // First pass of a plan with a folded layout change: reads `psi` in the
 // source layout, writes `out` in the pass layout (complex64, one vector).
+// Chunk bits of pass `pd` sorted by their position under `map` (state bit
+// position -> position in the other layout): positions ascending (pos) and
+// the chunk bit of each (chunk); comp[m]: position of block-index bit m.
+void remap_bits(const PassDev &pd, const int *map, int *pos, int *chunk, int *comp)
+{
+    std::vector<std::pair<int, int>> v;
+    for (int j = 0; j < pd.K; ++j) v.push_back({map[pd.pos[j]], j});
+    std::sort(v.begin(), v.end());
+    for (int k = 0; k < pd.K; ++k) pos[k] = v[k].first, chunk[k] = v[k].second;
+    for (int m = 0; m < pd.ncomp; ++m) comp[m] = map[pd.comp[m]];
+}
+
+// A pass that reads its input in another layout (a folded layout change,
+// Plan::in_map, first pass) and/or writes its output in another layout (the
+// output order, Plan::out_map, last pass), from `psi` into `out`, through the
+// specialised copies on both sides.  Empty when they do not apply.
+std::string remapped_pass_source(const Plan &plan, size_t p, bool dbl, const std::string &name)
+{
+    const PassDev &pd = plan.passes[p];
+    const char *T = dbl ? "double" : "float";
+    const int n = pd.K + pd.ncomp;
+    const int K = pd.K, E = 1 << K;
+    const int TPB = jit_block_threads(K, 1);
+    int ident[64], inv[64], Pin[64], Jin[64], Cin[64], Pout[64], Jout[64], Cout[64];
+    for (int k = 0; k < 64; ++k) ident[k] = inv[k] = k;
+    if (p == 0 && !plan.in_map.empty())
+        for (int q = 0; q < n; ++q) inv[plan.in_map[q]] = q;
+    remap_bits(pd, inv, Pin, Jin, Cin);
+    const bool wmap = p + 1 == plan.passes.size() && !plan.out_map.empty();
+    remap_bits(pd, wmap ? plan.out_map.data() : ident, Pout, Jout, Cout);
+    int cfi = 0, cfo = 0;
+    while (cfi < K && Pin[cfi] == cfi) ++cfi;
+    while (cfo < K && Pout[cfo] == cfo) ++cfo;
+    LaneTable lt;
+    const CopyPlan ci = copy_plan(lt, K, dbl, TPB, Pin, Jin, cfi), co = copy_plan(lt, K, dbl, TPB, Pout, Jout, cfo);
+    if (!ci.ok || !co.ok) return std::string();
+    std::ostringstream o, b;
+    o << "extern \"C\" __global__ __launch_bounds__(" << TPB << ")"
+      << (dbl ? "" : " __attribute__((amdgpu_waves_per_eu(4)))") << "\nvoid " << name << "(const cx<" << T
+      << "> *__restrict__ psi, cx<" << T << "> *__restrict__ out, const cx<" << T << "> *__restrict__ mats)\n{\n";
+    o << "  __shared__ __align__(16) cx<" << T << "> c0[" << E << "];\n";
+    emit_lane_prologue(b, name);
+    emit_copy_thread(b, ci, "i", Pin, Jin);
+    emit_copy_thread(b, co, "o", Pout, Jout);
+    b << "  const uint64_t c = xcd_slab(blockIdx.x, " << pd.ncomp << ");\n";
+    b << "  const uint64_t base_in = 0ull" << bit_moves("c", Cin, pd.ncomp, "uint64_t") << ";\n";
+    b << "  const uint64_t base_out = 0ull" << bit_moves("c", Cout, pd.ncomp, "uint64_t") << ";\n";
+    emit_copy(b, ci, "i", "psi", "c0", "base_in", Pin, Jin, K, true, dbl);
+    b << "  __syncthreads();\n";
+    for (int s = 0; s < pd.nsteps; ++s) emit_step(b, plan.steps[pd.step_off + s], E, T, 1, TPB);
+    emit_copy(b, co, "o", "out", "c0", "base_out", Pout, Jout, K, false, dbl);
+    return lane_table_source(lt, name) + o.str() + b.str() + "}\n\n";
+}
+
 std::string mapped_pass_source(const Plan &plan, bool dbl, const std::string &name)
 {
+    const std::string fast = remapped_pass_source(plan, 0, dbl, name);
+    if (!fast.empty()) return fast;
     const PassDev &pd = plan.passes[0];
     const char *T = dbl ? "double" : "float";
     const int n = (int)plan.in_map.size();
@@ -426,24 +487,6 @@ std::string mapped_pass_source(const Plan &plan, bool dbl, const std::string &na
       << (dbl ? "" : " __attribute__((amdgpu_waves_per_eu(4)))") << "\nvoid " << name << "(const cx<" << T
       << "> *__restrict__ psi, cx<" << T << "> *__restrict__ out, const cx<" << T << "> *__restrict__ mats)\n{\n";
     o << "  __shared__ __align__(16) cx<" << T << "> c0[" << E << "];\n";
-    int ident[64];
-    for (int k = 0; k < 64; ++k) ident[k] = k;
-    LaneTable lt;
-    const CopyPlan ci = copy_plan(lt, K, dbl, TPB, Pold, J, CFR), co = copy_plan(lt, K, dbl, TPB, pd.pos, ident, CF);
-    if (ci.ok && co.ok) {
-        std::ostringstream b;
-        emit_lane_prologue(b, name);
-        emit_copy_thread(b, ci, "i", Pold, J);
-        emit_copy_thread(b, co, "o", pd.pos, ident);
-        b << "  const uint64_t c = xcd_slab(blockIdx.x, " << pd.ncomp << ");\n";
-        b << "  const uint64_t base = 0ull" << bit_moves("c", pd.comp, pd.ncomp, "uint64_t") << ";\n";
-        b << "  const uint64_t base_in = 0ull" << bit_moves("c", comp_old, pd.ncomp, "uint64_t") << ";\n";
-        emit_copy(b, ci, "i", "psi", "c0", "base_in", Pold, J, K, true, dbl);
-        b << "  __syncthreads();\n";
-        for (int s = 0; s < pd.nsteps; ++s) emit_step(b, plan.steps[pd.step_off + s], E, T, 1, TPB);
-        emit_copy(b, co, "o", "out", "c0", "base", pd.pos, ident, K, false, dbl);
-        return lane_table_source(lt, name) + o.str() + b.str() + "}\n\n";
-    }
     o << "  __shared__ uint64_t rowoff[" << NROW << "];\n  __shared__ uint64_t rowin[" << NROWIN << "];\n";
     o << "  for (int t = threadIdx.x; t < " << NROW << "; t += " << TPB << ")\n    rowoff[t] = 0ull"
       << bit_moves("t", pd.pos + CF, K - CF, "uint64_t") << ";\n";
@@ -468,6 +511,15 @@ std::string mapped_pass_source(const Plan &plan, bool dbl, const std::string &na
 
 }  // namespace
 
+bool jit_mapped_last(const Plan &plan, bool dbl, int nv)
+{
+    if (plan.out_map.empty() || nv != 1 || plan.passes.empty()) return false;
+    const size_t last = plan.passes.size() - 1;
+    if (last == 0 && !plan.in_map.empty() && !jit_mapped_first(plan, dbl, nv)) return false;
+    if (last == 0 && plan.zero_input) return false;
+    return !remapped_pass_source(plan, last, dbl, "@").empty();
+}
+
 bool jit_mapped_first(const Plan &plan, bool dbl, int nv)
 {
     if (plan.in_map.empty() || nv != 1 || plan.passes.empty()) return false;
@@ -482,6 +534,8 @@ bool jit_mapped_first(const Plan &plan, bool dbl, int nv)
 
 std::string jit_pass_source(const Plan &plan, size_t p, bool dbl, int nv, const std::string &name)
 {
+    if (p + 1 == plan.passes.size() && jit_mapped_last(plan, dbl, nv))
+        return remapped_pass_source(plan, p, dbl, name);
     if (p == 0 && jit_mapped_first(plan, dbl, nv)) return mapped_pass_source(plan, dbl, name);
     const PassDev &pd = plan.passes[p];
     const char *T = dbl ? "double" : "float";

@@ -46,4 +46,10 @@ int jit_block_threads(int K, int nv);
 // and writing the pass layout to the other buffer (complex64, one vector).
 bool jit_mapped_first(const Plan &plan, bool dbl, int nv);
 
+// Whether the last pass of `plan` writes the output order (Plan::out_map):
+// its kernel is then kernel(psi_in, psi_out, mats), writing the other buffer
+// in that order (reading the source layout too when it is also the first
+// pass of a plan with Plan::in_map; one vector).
+bool jit_mapped_last(const Plan &plan, bool dbl, int nv);
+
 }  // namespace qsim

@@ -506,7 +506,8 @@ void to_dense(Ctx &c, StepBuilder &sb)
 }  // namespace
 
 bool plan_passes(int n, const PlanOp *ops, int nops, const void *mats, int elem_bytes, int K, int cmin,
-                 int max_probes, Plan &out, std::string &err, int fuse, const int32_t *in_map)
+                 int max_probes, Plan &out, std::string &err, int fuse, const int32_t *in_map,
+                 const int32_t *out_map)
 {
     out = Plan();
     if (n < kG || n > 63) {
@@ -561,6 +562,19 @@ bool plan_passes(int n, const PlanOp *ops, int nops, const void *mats, int elem_
         }
         out.in_map.assign(in_map, in_map + n);
     }
+    uint64_t out_rows = 0;  // pass-layout positions that become the output's row bits
+    if (out_map) {
+        uint64_t seen = 0;
+        for (int p = 0; p < n; ++p) {
+            if (out_map[p] < 0 || out_map[p] >= n || ((seen >> out_map[p]) & 1)) {
+                err = "tile plan: output map is not a permutation";
+                return false;
+            }
+            seen |= 1ull << out_map[p];
+            if (out_map[p] < std::max(cmin, 1)) out_rows |= 1ull << p;
+        }
+    }
+    bool out_folded = false;
     bool first = true;  // a layout change with no op still gets its (stepless) pass
     while (!rem.empty() || (first && in_map)) {
         // ---- pass: chunk bit set ----
@@ -594,6 +608,11 @@ bool plan_passes(int n, const PlanOp *ops, int nops, const void *mats, int elem_
             P |= m;
             nprobe += probe;
             chosen.push_back(idx);
+        }
+        if (out_map && left.empty() && !out_folded && popc(P | out_rows) <= Kp) {
+            // the last pass: it also writes the output order, in rows
+            P |= out_rows;
+            out_folded = true;
         }
         P = fill_low(P, n, Kp);
         PassDev pd;
@@ -703,6 +722,7 @@ bool plan_passes(int n, const PlanOp *ops, int nops, const void *mats, int elem_
         out.passes.push_back(pd);
         rem.swap(left);
     }
+    if (out_folded) out.out_map.assign(out_map, out_map + n);
     // the remaining factor goes to the last step
     if (!(pending.re == 1.0 && pending.im == 0.0) && !out.steps.empty()) {
         StepDev &last = out.steps.back();

@@ -52,6 +52,12 @@ struct Plan {
     // then reads the input in the old layout and writes its output, in the
     // pass layout, to another buffer.
     std::vector<int32_t> in_map;
+    // When set (a layout change folded into the last pass): out_map[p] is the
+    // position in the output of pass-layout bit position p.  The last pass
+    // then writes its output in that order to another buffer; it holds the
+    // preimages of the output's lowest cmin positions, so its writes run in
+    // rows.  Left empty by plan_passes when the last pass has no room for them.
+    std::vector<int32_t> out_map;
     // When set (the input is |0...0>, not yet written): the first pass
     // generates its chunks instead of reading them.
     bool zero_input = false;
@@ -75,9 +81,11 @@ struct Plan {
 // in_map (n entries, or null): the input is in another layout (see
 // Plan::in_map); the first pass also holds the images of the lowest cmin
 // input bit positions, so it reads runs of 2^cmin contiguous amplitudes.
+// out_map (n entries, or null): the output goes to another layout (see
+// Plan::out_map), folded into the last pass when it has room.
 bool plan_passes(int n, const PlanOp *ops, int nops, const void *mats, int elem_bytes, int K,
                  int cmin, int max_probes, Plan &out, std::string &err, int fuse = 0,
-                 const int32_t *in_map = nullptr);
+                 const int32_t *in_map = nullptr, const int32_t *out_map = nullptr);
 
 // Pauli chunk passes: each term's X support must lie inside its pass's chunk.
 struct PauliPass {

@@ -120,6 +120,19 @@ int qsim_validate_ops(int num_qubits, const qsim_op *ops, int nops, uint64_t mat
 int qsim_apply_ops(qsim_state *st, int mode, const qsim_op *ops, int nops, const void *mats,
                    uint64_t mat_count);
 
+/* apply_tile_* followed by permute(layout_old, layout_new): no reference
+ * counterpart as one call; it replaces the pair of calls the tile loop of
+ * run_custom_backend makes for its last tile with ops and the layout changes
+ * after it (qgpusim/backends/custom/backend.py:417-473; bindings.cpp:434-483).
+ * Results, errors and the state afterwards are those of qsim_apply_ops then
+ * qsim_permute.  In a chunked mode with specialised kernels the last chunk
+ * pass writes its output in the new layout into the second buffer (no
+ * separate permutation pass), when that pass has room for the chunk bits
+ * that become the output's contiguous rows; otherwise the change is recorded
+ * as by qsim_permute. */
+int qsim_apply_ops_permute(qsim_state *st, int mode, const qsim_op *ops, int nops, const void *mats,
+                           uint64_t mat_count, const int32_t *layout_old, const int32_t *layout_new, int num_qubits);
+
 /* apply_1q / apply_2q with a host matrix (bindings.cpp:499-544). */
 int qsim_apply_1q(qsim_state *st, int target_qubit, const void *U);
 int qsim_apply_2q(qsim_state *st, int q0, int q1, const void *U);
@@ -180,12 +193,15 @@ int qsim_set_jit(qsim_state *st, int enable);
 /* Host only, no reference counterpart: plan `ops` as qsim_apply_ops would (as
  * the adjoint's reverse sweep would, on psi and lam, when `ops` hold
  * QSIM_OP_PROBE entries; with a pending layout change in_map[p] = new
- * position of bit p, forward only, or NULL) and compile, without
- * loading, every specialised pass kernel for gfx950.  Returns the number of
- * passes, or -1 with the compiler log. */
+ * position of bit p, forward only, or NULL; with a layout change after the
+ * list out_map[p] = output position of bit p, as qsim_apply_ops_permute
+ * would, or NULL) and compile, without loading, every specialised pass
+ * kernel for gfx950.  *out_folded (may be NULL) tells whether the last pass
+ * writes the out_map order.  Returns the number of passes, or -1 with the
+ * compiler log. */
 int qsim_jit_compile_check(int num_qubits, int dtype, const qsim_op *ops, int nops,
                            const void *mats, uint64_t mat_count, int K, int cmin, int fuse,
-                           const int32_t *in_map);
+                           const int32_t *in_map, const int32_t *out_map, int *out_folded);
 
 /* Host only, no reference counterpart: the stored bit order an op list would
  * run on (the layout change its first chunk pass folds in).  perm_map[p] is
@@ -242,14 +258,16 @@ int qsim_profile_read(qsim_state *st, double *ms_out, int64_t *count_out, int nc
  * Returns the number written. */
 int qsim_get_counters(const qsim_state *st, int64_t *out, int n);
 
-/* Host only: plan `ops` into chunk passes without a device.  Writes, per pass,
- * [K, cfast, nsteps, pos[0..K-1]] followed, per step, by [register bits 0..3,
- * rounds, dense] into `desc` (capacity desc_cap ints), and the
- * op order of execution into `order` (nops entries; each op index once).
- * Returns the number of passes, or -1. */
+/* Host only: plan `ops` into chunk passes without a device, after a pending
+ * layout change in_map (stored bit p moves to in_map[p]; null for none) and
+ * before one out_map (null for none), as the chunked modes plan them.
+ * Writes, per pass, [K, cfast, nsteps, pos[0..K-1]] followed, per step, by
+ * [register bits 0..3, rounds, dense] into `desc` (capacity desc_cap ints),
+ * and the op order of execution into `order` (nops entries; each op index
+ * once).  Returns the number of passes, or -1. */
 int qsim_plan_describe(int num_qubits, int dtype, const qsim_op *ops, int nops, const void *mats,
-                       uint64_t mat_count, int K, int cmin, int32_t *desc, int desc_cap,
-                       int32_t *order);
+                       uint64_t mat_count, int K, int cmin, const int32_t *in_map, const int32_t *out_map,
+                       int32_t *desc, int desc_cap, int32_t *order);
 
 /* ---- Batched circuits (no reference counterpart: the reference runs one
  * circuit per call).  Many independent circuits run in one persistent

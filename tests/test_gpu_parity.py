@@ -744,6 +744,59 @@ def test_stored_layout_with_few_row_bits(hip, dtype, cmin):
     assert (c["folded_permutations"] >= 1) if folds else (c["folded_permutations"] == 0 and c["permutations"] >= 1)
 
 
+_TILE_MODES = {"apply_tile_dev": 0, "apply_tile_graphed": 1, "apply_tile_cooperative": 2}
+
+
+@pytest.mark.parametrize("dtype", [np.complex64, np.complex128])
+@pytest.mark.parametrize("jit", [True, False])
+@pytest.mark.parametrize("mode", ["apply_tile_cooperative", "apply_tile_graphed", "apply_tile_dev"])
+@pytest.mark.parametrize("n,width,depth,folds", [(14, 14, 3, 1), (20, 12, 3, 2), (20, 8, 2, 2)])
+def test_tile_then_layout_change_in_one_call(hip, dtype, jit, mode, n, width, depth, folds):
+    """apply_tile_permute(mode, ops, old, new) equals the tile followed by
+    permute(old, new).  With specialised chunk passes the last pass writes
+    the new order and no permutation pass runs: at 14 qubits (two passes;
+    the pending order needs no move), at 20 qubits over two passes (the
+    first also reads the pending order) and in one pass that does both.  A
+    reset state (first pass generates |0...0>) and graph replays included."""
+    rng = np.random.default_rng(23 + n)
+    L = _layouts(n, rng, 2)
+    T = [int(q) for q in rng.permutation(n)[:width]]
+    ops = [(o[0], *[T[q] for q in o[1:-1]], o[-1]) for o in _tuples(C.random_circuit(width, depth, 31 + n))]
+    psi0 = C.random_state(n, 14)
+    ref = P.permute(psi0, L[0], L[1])
+    for o in ops:
+        ref = dense.apply(ref, o[-1], list(o[1:-1]), n)
+    ref = P.permute(ref, L[1], L[2])
+    e0 = np.zeros(1 << n, dtype=np.complex128)
+    e0[0] = 1
+    ref0 = e0
+    for o in ops:
+        ref0 = dense.apply(ref0, o[-1], list(o[1:-1]), n)
+    ref0 = P.permute(ref0, L[0], L[1])
+    sv = _sv(hip, n, dtype)
+    sv.set_jit(jit)
+    print(f"tile_permute n={n} width={width} dtype={np.dtype(dtype).name} jit={jit} mode={mode}", flush=True)
+    for rep in range(3):
+        sv.from_numpy(psi0)
+        sv.permute(L[0], L[1])
+        sv.apply_tile_permute(_TILE_MODES[mode], ops, L[1], L[2])
+        _check(ref, sv.to_numpy(), dtype)
+    c = sv.counters()
+    if jit and mode != "apply_tile_dev":
+        assert c["permutations"] == 0
+        assert c["folded_permutations"] == 3 * folds
+        if mode == "apply_tile_graphed":
+            assert c["graph_launches"] > c["graph_captures"]
+    else:
+        assert c["folded_permutations"] == 0 and c["permutations"] >= 3
+    for rep in range(2):
+        sv.reset()
+        sv.apply_tile_permute(_TILE_MODES[mode], ops, L[0], L[1])
+        _check(ref0, sv.to_numpy(), dtype)
+    with pytest.raises(RuntimeError):
+        sv.apply_tile_permute(_TILE_MODES[mode], ops, L[0], L[0][:-1])
+
+
 @pytest.mark.parametrize("dtype", [np.complex64, np.complex128])
 @pytest.mark.parametrize("jit", [True, False])
 @pytest.mark.parametrize("mode", ["apply_tile_cooperative", "apply_tile_graphed", "apply_tile_dev"])

@@ -358,6 +358,12 @@ class _OracleState:
 
     apply_tile_cooperative = apply_tile_graphed = apply_tile_dev = _apply
 
+    def apply_tile_permute(self, mode, ops, layout_old, layout_new):
+        # the native call runs the tile in the given mode, then the layout change
+        run = {0: self.apply_tile_dev, 1: self.apply_tile_graphed, 2: self.apply_tile_cooperative}[mode]
+        run(ops)
+        self.permute(layout_old, layout_new)
+
     def synchronize(self):
         pass
 
@@ -823,8 +829,10 @@ def test_random33_tiles_plan_in_two_passes(hip):
     13-bit chunks at least) plans in exactly two passes."""
     import bench
     qc, plan, prog, plain, ng, tt = bench.build_program(hip, 33, 30, 18, 1002)
-    counts = [len(hip.plan_describe(33, it[1], it[2], it[3], 13, 4)[0]) for it in prog if it[0] == "tile"]
+    counts = [len(hip.plan_describe(33, it[1], it[2], it[3], 13, 4)[0]) for it in prog
+              if it[0] in ("tile", "tile_perm")]
     assert len(counts) == 19 and all(c == 2 for c in counts)
+    assert prog[-1][0] == "tile_perm"  # the final layout change rides on the last tile
 
 
 def test_chunk_plan_keeps_probe_order(hip):
@@ -1073,3 +1081,52 @@ def test_result_rows_follow_reference_schema(tmp_path):
     assert rows[1]["tvd"] == "0.030000" and rows[1]["passed"] == "False" and rows[1]["shots"] == "1024"
     kl = 0.75 * np.log(3) + 0.25 * np.log(1 / 3)
     assert abs(float(rows[1]["kl_div"]) - kl) < 1e-6
+
+
+def _replay_stored_order(hip, n, prog, dtype=np.complex64):
+    """Replays the library's stored-order bookkeeping over a bench program on
+    the host: yields (item, ops mapped to the stored order, in_map or None,
+    stored order after the list)."""
+    perm, pending = list(range(n)), False
+    for item in prog:
+        if item[0] == "perm":
+            m = hip.permute_map(item[1], item[2])
+            perm = [m[perm[p]] for p in range(n)] if pending else m
+            pending = perm != list(range(n))
+            continue
+        arr, nops, arena = item[1], item[2], item[3]
+        in_map, after, mapped = None, list(range(n)), arr
+        if pending:
+            tgt = hip.choose_layout(n, perm, arr, nops, dtype)
+            mapped = (hip.QsimOp * nops)()
+            for k in range(nops):
+                o = arr[k]
+                mapped[k].kind, mapped[k].q0, mapped[k].mat_offset = o.kind, tgt[o.q0], o.mat_offset
+                mapped[k].q1 = tgt[o.q1] if o.kind == hip.OP_2Q else -1
+            in_map = [tgt[perm[p]] for p in range(n)]
+            after = [0] * n
+            for p in range(n):
+                after[tgt[p]] = p
+        yield item, mapped, in_map, after
+        perm, pending = after, after != list(range(n))
+
+
+def test_final_layout_change_folds_into_last_pass(hip):
+    """The headline program (33 qubits, depth 30, nL = 18) hands its last tile
+    and the layout changes after it to one call, and that list's last chunk
+    pass has room for the output's row bits: the final restore needs no
+    permutation pass, and the folded kernels compile."""
+    import bench
+    n = 33
+    _, _, prog, _, _, _ = bench.build_program(hip, n, 30, 18, 1002, np.complex64)
+    assert [p[0] for p in prog].count("tile_perm") == 1 and prog[-1][0] == "tile_perm"
+    assert sum(1 if p[0] == "perm" else p[6] if p[0] == "tile_perm" else 0 for p in prog) == 19
+    for item, mapped, in_map, after in _replay_stored_order(hip, n, prog):
+        if item[0] != "tile_perm":
+            continue
+        pm = hip.permute_map(item[4], item[5])
+        out_map = [pm[after[p]] for p in range(n)]
+        folded = []
+        npass = hip.jit_compile_check(n, mapped, item[2], item[3], 13, 4, in_map=in_map, out_map=out_map,
+                                      folded=folded)
+        assert npass == 2 and folded == [True]
