@@ -27,6 +27,7 @@ LIB_PATH = os.environ.get("QGPUSIM_LIB") or os.path.join(_PKG, "lib", "libqgpusi
 
 C64, C128 = 0, 1
 SEQUENTIAL, GRAPHED, COOPERATIVE = 0, 1, 2
+NOALLOC_SCRATCH_BYTES = 32 << 20  # QSIM_NOALLOC_SCRATCH_BYTES
 OP_1Q, OP_2Q, OP_PROBE = 1, 2, 3
 
 
@@ -91,6 +92,7 @@ _SIGNATURES = {
     "qsim_set_fusion": ([_P, _I], _I),
     "qsim_device_memory": ([_P, _P], _I),
     "qsim_sample": ([_P, _U64, _U64, _P], _I),
+    "qsim_sample_noalloc": ([_P, _U64, _U64, _P], _I),
 }
 
 EXPORTED = tuple(_SIGNATURES)
@@ -387,6 +389,15 @@ class _StatevectorBase:
         indices in the current bit order, grouped by index block."""
         out = np.zeros(max(1, int(shots)), dtype=np.uint64)
         _check(lib.qsim_sample(self._h, int(shots), int(seed) & (2 ** 64 - 1), out.ctypes.data))
+        return out[:int(shots)]
+
+    def sample_noalloc(self, shots, seed=0):
+        """sample() without device allocation: the library's fixed per-GPU
+        scratch (NOALLOC_SCRATCH_BYTES) holds its buffers; requests that do
+        not fit, or a pending layout change on a state without its second
+        buffer, are refused."""
+        out = np.zeros(max(1, int(shots)), dtype=np.uint64)
+        _check(lib.qsim_sample_noalloc(self._h, int(shots), int(seed) & (2 ** 64 - 1), out.ctypes.data))
         return out[:int(shots)]
 
     def set_fusion(self, mode):

@@ -19,7 +19,7 @@ ARCH = "gfx950"
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 
 SOURCES = ["qsim_kernels.hip", "qsim_capi.cpp", "qsim_plan.cpp", "qsim_jit.cpp", "qsim_batch.hip",
-           "qsim_batch.cpp", "qsim_sample.hip"]
+           "qsim_batch.cpp", "qsim_sample.hip", "qsim_noalloc.hip"]
 # device code that the run-time compiler (hiprtc) gets as one string
 PRELUDE_HEADERS = ["qsim_dev_types.h", "qsim_device.h"]
 PRELUDE_INC = os.path.join(CSRC, "qsim_jit_prelude.inc")

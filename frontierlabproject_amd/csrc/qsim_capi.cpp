@@ -930,6 +930,109 @@ int adjoint_t(qsim_state *st, const qsim_pauli *terms, int nterms, const qsim_op
 // ============================================================================
 // exported C ABI
 // ============================================================================
+namespace {
+size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
+}  // namespace
+
+// Sampling (qsim_sample).  Device memory: the block sums, the shot groups,
+// the residuals and the indices; allocated per call when `scratch` is null,
+// else carved from [scratch, scratch + scratch_bytes), which is then the only
+// device memory the call touches besides the state (refused beforehand when
+// too small; nothing in it needs to start at zero).
+int qsim::sample_impl(qsim_state *st, uint64_t shots, uint64_t seed, uint64_t *indices_out, void *scratch,
+                      size_t scratch_bytes)
+{
+    if (!st) return fail("null state");
+    if (shots == 0) return 0;
+    if (!indices_out) return fail("sample: null output");
+    if (shots > (1ull << 31)) return fail("sample: at most 2^31 shots per call");
+    const uint64_t nb = (st->dim + (1ull << kSampleB) - 1) >> kSampleB;
+    const size_t blk_bytes = align256(nb * sizeof(double));
+    const size_t rbytes = shots * sizeof(double);
+    if (scratch) {
+        const size_t need = blk_bytes + align256(std::min<uint64_t>(shots, nb) * sizeof(SampleGroup)) +
+                            align256(rbytes) + shots * sizeof(uint64_t);
+        if (need > scratch_bytes)
+            return fail("sample_noalloc: " + std::to_string(shots) + " shots of a " + std::to_string(st->n) +
+                        "-qubit state need " + std::to_string(need) + " bytes of scratch; the library holds " +
+                        std::to_string(scratch_bytes));
+        // a pending layout change is applied through the second buffer
+        if (st->perm_pending && !st->tmp)
+            return fail("sample_noalloc: the state holds a pending layout change and has no second buffer to "
+                        "apply it with (call qsim_sample once, or read the state, first)");
+    }
+    if (flush_perm(st)) return -1;
+    double *d_blk = nullptr;
+    std::unique_ptr<void, void (*)(void *)> g1(nullptr, [](void *p) { hipFree(p); });
+    if (scratch) {
+        d_blk = static_cast<double *>(scratch);
+    } else {
+        HIP_TRY(hipMalloc(reinterpret_cast<void **>(&d_blk), nb * sizeof(double)));
+        g1.reset(d_blk);
+    }
+    if (st->dtype == QSIM_C64)
+        HIP_TRY(launch_block_prob<float>(static_cast<cx<float> *>(st->psi), st->dim, d_blk, st->stream));
+    else
+        HIP_TRY(launch_block_prob<double>(static_cast<cx<double> *>(st->psi), st->dim, d_blk, st->stream));
+    std::vector<double> cdf(nb);
+    HIP_TRY(hipMemcpyAsync(cdf.data(), d_blk, nb * sizeof(double), hipMemcpyDeviceToHost, st->stream));
+    HIP_TRY(hipStreamSynchronize(st->stream));
+    for (uint64_t b = 1; b < nb; ++b) cdf[b] += cdf[b - 1];
+    const double total = cdf[nb - 1];
+    if (!(total > 0.0)) return fail("sample: the state has zero norm");
+    // shot k: u = U(k) * total with U from splitmix64 of (seed, k), 53 bits
+    std::vector<std::pair<uint64_t, double>> draws(shots);  // (block, residual)
+    for (uint64_t k = 0; k < shots; ++k) {
+        uint64_t z = seed + 0x9E3779B97F4A7C15ull * (k + 1);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        z ^= z >> 31;
+        const double u = (double)(z >> 11) * 0x1p-53 * total;
+        uint64_t b = (uint64_t)(std::upper_bound(cdf.begin(), cdf.end(), u) - cdf.begin());
+        // u rounded up to the total: the last block with nonzero probability
+        if (b >= nb) b = (uint64_t)(std::lower_bound(cdf.begin(), cdf.end(), total) - cdf.begin());
+        draws[k] = {b, u - (b ? cdf[b - 1] : 0.0)};
+    }
+    std::sort(draws.begin(), draws.end());
+    std::vector<SampleGroup> groups;
+    std::vector<double> resid(shots);
+    for (uint64_t k = 0; k < shots; ++k) {
+        resid[k] = draws[k].second;
+        if (groups.empty() || groups.back().block != draws[k].first)
+            groups.push_back(SampleGroup{draws[k].first, (uint32_t)k, 0});
+        groups.back().count++;
+    }
+    const size_t gbytes = align256(groups.size() * sizeof(SampleGroup));
+    char *base = nullptr;
+    std::unique_ptr<void, void (*)(void *)> g2(nullptr, [](void *p) { hipFree(p); });
+    if (scratch) {
+        base = static_cast<char *>(scratch) + blk_bytes;
+    } else {
+        void *d_buf = nullptr;
+        HIP_TRY(hipMalloc(&d_buf, gbytes + align256(rbytes) + shots * sizeof(uint64_t)));
+        g2.reset(d_buf);
+        base = static_cast<char *>(d_buf);
+    }
+    SampleGroup *d_groups = reinterpret_cast<SampleGroup *>(base);
+    double *d_resid = reinterpret_cast<double *>(base + gbytes);
+    uint64_t *d_idx = reinterpret_cast<uint64_t *>(base + gbytes + align256(rbytes));
+    HIP_TRY(hipMemcpyAsync(d_groups, groups.data(), groups.size() * sizeof(SampleGroup), hipMemcpyHostToDevice,
+                           st->stream));
+    HIP_TRY(hipMemcpyAsync(d_resid, resid.data(), rbytes, hipMemcpyHostToDevice, st->stream));
+    if (st->dtype == QSIM_C64)
+        HIP_TRY(launch_sample_blocks<float>(static_cast<cx<float> *>(st->psi), st->dim, d_groups, (int)groups.size(),
+                                            d_resid, d_idx, st->stream));
+    else
+        HIP_TRY(launch_sample_blocks<double>(static_cast<cx<double> *>(st->psi), st->dim, d_groups,
+                                             (int)groups.size(), d_resid, d_idx, st->stream));
+    HIP_TRY(hipMemcpyAsync(indices_out, d_idx, shots * sizeof(uint64_t), hipMemcpyDeviceToHost, st->stream));
+    HIP_TRY(hipStreamSynchronize(st->stream));
+    st->counters[0] += 2;
+    return 0;
+}
+
+int qsim::state_device(const qsim_state *st) { return st ? st->device : -1; }
+
 extern "C" {
 
 const char *qsim_last_error(void) { return g_err.c_str(); }
@@ -1353,69 +1456,7 @@ int qsim_choose_layout(int num_qubits, int dtype, int cmin, const int32_t *perm_
 
 int qsim_sample(qsim_state *st, uint64_t shots, uint64_t seed, uint64_t *indices_out)
 {
-    if (!st) return fail("null state");
-    if (shots == 0) return 0;
-    if (!indices_out) return fail("sample: null output");
-    if (shots > (1ull << 31)) return fail("sample: at most 2^31 shots per call");
-    if (flush_perm(st)) return -1;
-    const uint64_t nb = (st->dim + (1ull << kSampleB) - 1) >> kSampleB;
-    double *d_blk = nullptr;
-    HIP_TRY(hipMalloc(reinterpret_cast<void **>(&d_blk), nb * sizeof(double)));
-    std::unique_ptr<void, void (*)(void *)> g1(d_blk, [](void *p) { hipFree(p); });
-    if (st->dtype == QSIM_C64)
-        HIP_TRY(launch_block_prob<float>(static_cast<cx<float> *>(st->psi), st->dim, d_blk, st->stream));
-    else
-        HIP_TRY(launch_block_prob<double>(static_cast<cx<double> *>(st->psi), st->dim, d_blk, st->stream));
-    std::vector<double> cdf(nb);
-    HIP_TRY(hipMemcpyAsync(cdf.data(), d_blk, nb * sizeof(double), hipMemcpyDeviceToHost, st->stream));
-    HIP_TRY(hipStreamSynchronize(st->stream));
-    for (uint64_t b = 1; b < nb; ++b) cdf[b] += cdf[b - 1];
-    const double total = cdf[nb - 1];
-    if (!(total > 0.0)) return fail("sample: the state has zero norm");
-    // shot k: u = U(k) * total with U from splitmix64 of (seed, k), 53 bits
-    std::vector<std::pair<uint64_t, double>> draws(shots);  // (block, residual)
-    for (uint64_t k = 0; k < shots; ++k) {
-        uint64_t z = seed + 0x9E3779B97F4A7C15ull * (k + 1);
-        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
-        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
-        z ^= z >> 31;
-        const double u = (double)(z >> 11) * 0x1p-53 * total;
-        uint64_t b = (uint64_t)(std::upper_bound(cdf.begin(), cdf.end(), u) - cdf.begin());
-        // u rounded up to the total: the last block with nonzero probability
-        if (b >= nb) b = (uint64_t)(std::lower_bound(cdf.begin(), cdf.end(), total) - cdf.begin());
-        draws[k] = {b, u - (b ? cdf[b - 1] : 0.0)};
-    }
-    std::sort(draws.begin(), draws.end());
-    std::vector<SampleGroup> groups;
-    std::vector<double> resid(shots);
-    for (uint64_t k = 0; k < shots; ++k) {
-        resid[k] = draws[k].second;
-        if (groups.empty() || groups.back().block != draws[k].first)
-            groups.push_back(SampleGroup{draws[k].first, (uint32_t)k, 0});
-        groups.back().count++;
-    }
-    void *d_buf = nullptr;
-    const size_t gbytes = (groups.size() * sizeof(SampleGroup) + 255) & ~size_t(255);
-    const size_t rbytes = shots * sizeof(double);
-    HIP_TRY(hipMalloc(&d_buf, gbytes + rbytes + shots * sizeof(uint64_t)));
-    std::unique_ptr<void, void (*)(void *)> g2(d_buf, [](void *p) { hipFree(p); });
-    char *base = static_cast<char *>(d_buf);
-    SampleGroup *d_groups = reinterpret_cast<SampleGroup *>(base);
-    double *d_resid = reinterpret_cast<double *>(base + gbytes);
-    uint64_t *d_idx = reinterpret_cast<uint64_t *>(base + gbytes + rbytes);
-    HIP_TRY(hipMemcpyAsync(d_groups, groups.data(), groups.size() * sizeof(SampleGroup), hipMemcpyHostToDevice,
-                           st->stream));
-    HIP_TRY(hipMemcpyAsync(d_resid, resid.data(), rbytes, hipMemcpyHostToDevice, st->stream));
-    if (st->dtype == QSIM_C64)
-        HIP_TRY(launch_sample_blocks<float>(static_cast<cx<float> *>(st->psi), st->dim, d_groups, (int)groups.size(),
-                                            d_resid, d_idx, st->stream));
-    else
-        HIP_TRY(launch_sample_blocks<double>(static_cast<cx<double> *>(st->psi), st->dim, d_groups,
-                                             (int)groups.size(), d_resid, d_idx, st->stream));
-    HIP_TRY(hipMemcpyAsync(indices_out, d_idx, shots * sizeof(uint64_t), hipMemcpyDeviceToHost, st->stream));
-    HIP_TRY(hipStreamSynchronize(st->stream));
-    st->counters[0] += 2;
-    return 0;
+    return qsim::sample_impl(st, shots, seed, indices_out, nullptr, 0);
 }
 
 int qsim_device_memory(uint64_t *free_bytes, uint64_t *total_bytes)

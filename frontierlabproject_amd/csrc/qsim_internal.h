@@ -8,6 +8,8 @@
 #include "qsim_plan.h"
 #include "qsim_device.h"
 
+struct qsim_state;
+
 namespace qsim {
 
 template <typename R> struct Mat1 { cx<R> u[4]; };
@@ -75,5 +77,13 @@ hipError_t prepare_kernels();
 
 // Sets the thread's last-error string (qsim_last_error) and returns -1.
 int set_error(const std::string &msg);
+
+// qsim_sample's work; with `scratch` non-null every device buffer it needs
+// comes from [scratch, scratch + scratch_bytes) instead of hipMalloc (refused
+// beforehand when too small).
+int sample_impl(qsim_state *st, uint64_t shots, uint64_t seed, uint64_t *indices_out, void *scratch,
+                size_t scratch_bytes);
+// The device a state was created on (-1 for null).
+int state_device(const qsim_state *st);
 
 }  // namespace qsim

@@ -237,6 +237,26 @@ int qsim_set_fusion(qsim_state *st, int mode);
  * Applies a pending layout change first.  At most 2^31 shots per call. */
 int qsim_sample(qsim_state *st, uint64_t shots, uint64_t seed, uint64_t *indices_out);
 
+/* Device scratch held by the library for qsim_sample_noalloc, per GPU. */
+#define QSIM_NOALLOC_SCRATCH_BYTES (32u << 20)
+
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  qsim_sample with the same arguments and the same
+ * results, but it calls no hipMalloc or hipFree: its device buffers (block
+ * sums, shot groups, residuals, indices) come from a fixed array of
+ * QSIM_NOALLOC_SCRATCH_BYTES in the library, one per GPU, held until the
+ * process ends.  Needs 8 * ceil(2^n / 2^13) + 16 * min(shots, blocks) +
+ * 16 * shots bytes (each part rounded up to 256); larger requests are
+ * refused before any work.  A pending layout change is applied first, which
+ * needs the state's second buffer: a state without one is refused rather
+ * than given one.  Every call on a GPU shares that GPU's array, which keeps
+ * the last call's contents: the library runs the calls on one GPU one at a
+ * time (each returns after its kernels end); overlapping them on one GPU by
+ * any other route, e.g. another copy of the library in the same process,
+ * would race. */
+int qsim_sample_noalloc(qsim_state *st, uint64_t shots, uint64_t seed, uint64_t *indices_out);
+
 /* No reference counterpart (the reference reads NVML): free and total memory
  * of the current device (hipMemGetInfo). */
 int qsim_device_memory(uint64_t *free_bytes, uint64_t *total_bytes);

@@ -163,3 +163,14 @@ def test_usage_merge_rule():
         {"rss_mb": "2.00", "gpu_mem_mb": "", "gpu_util": ""}
     snap = R._mem_snapshot()
     assert set(snap) == {"rss_mb", "gpu_mem_mb", "gpu_util"} and float(snap["rss_mb"]) > 0
+
+
+def test_noalloc_scratch_size_matches_header(hip):
+    """The binding's scratch size is the one the C header declares."""
+    import os
+    import re
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    with open(os.path.join(root, "include", "qgpusim.h")) as f:
+        m = re.search(r"#define QSIM_NOALLOC_SCRATCH_BYTES \((\d+)u << (\d+)\)", f.read())
+    assert m and int(m.group(1)) << int(m.group(2)) == hip.NOALLOC_SCRATCH_BYTES
+    assert hasattr(hip.lib, "qsim_sample_noalloc")
