@@ -337,7 +337,7 @@ void emit_copy_thread(std::ostringstream &o, const CopyPlan &c, const char *tag,
 // stores (LDS -> global, the pass layout).  Accesses are issued in batches of
 // eight per thread, all loads of a batch before any of its LDS writes.
 void emit_copy(std::ostringstream &o, const CopyPlan &c, const char *tag, const char *g, const char *lds,
-               const char *base, const int *src, const int *chunk, int K, bool load, bool dbl)
+               const char *base, const int *src, const int *chunk, int K, bool load, bool dbl, bool zeros = false)
 {
     const int B = 8;
     const std::string t(tag);
@@ -371,7 +371,9 @@ void emit_copy(std::ostringstream &o, const CopyPlan &c, const char *tag, const 
                           << ".x, " << bq << ".y}; lds_at(" << lds << ", sl ^ " << s1 << ") = cx<float>{" << bq
                           << ".z, " << bq << ".w}; }\n";
                 } else if (pass == 0) {
-                    if (dbl || ident)
+                    if (zeros)
+                        o << "    " << bq << " = v4f{0.f, 0.f, 0.f, 0.f};\n";
+                    else if (dbl || ident)
                         o << "    " << bq << " = *" << la << ";\n";
                     else
                         o << "    { const int sl = " << slot << "; const cx<float> x0 = lds_at(" << lds
@@ -565,7 +567,12 @@ std::string jit_pass_source(const Plan &plan, size_t p, bool dbl, int nv, const 
         b << "  const uint64_t c = xcd_slab(blockIdx.x, " << pd.ncomp << ");\n";
         b << "  const uint64_t base = 0ull" << bit_moves("c", pd.comp, pd.ncomp, "uint64_t") << ";\n";
         if (p == 0 && plan.zero_input && nv == 1) {
-            // the input is |0...0>: the chunk holding amplitude 0 starts as e_0, every other as zeros
+            // the input is |0...0>: the chunk holding amplitude 0 starts as
+            // e_0, every other as zeros and stays zero under the pass's gates
+            // (they act within the chunk), so those blocks only store zeros
+            b << "  if (base != 0) {\n";
+            emit_copy(b, cp, "", "psi", "c0", "base", pd.pos, ident, K, false, dbl, true);
+            b << "    return;\n  }\n";
             b << "  for (int t = threadIdx.x; t < " << E << "; t += " << TPB << ") c0[t] = cx<" << T << ">{0, 0};\n";
             b << "  __syncthreads();\n";
             b << "  if (base == 0 && threadIdx.x == 0) c0[swz(0)] = cx<" << T << ">{1, 0};\n";
@@ -604,10 +611,13 @@ std::string jit_pass_source(const Plan &plan, size_t p, bool dbl, int nv, const 
     o << "  __syncthreads();\n";
     const std::string ntf = ", true, " + std::to_string(TPB);  // non-temporal (streamed once), block size
     if (p == 0 && plan.zero_input && nv == 1) {
-        // the input is |0...0>: the chunk holding amplitude 0 starts as e_0, every other as zeros
+        // the input is |0...0>: the chunk holding amplitude 0 starts as e_0,
+        // every other as zeros and stays zero (those blocks store zeros only)
         o << "  for (int t = threadIdx.x; t < " << E << "; t += " << TPB << ") c0[t] = cx<" << T << ">{0, 0};\n";
         o << "  __syncthreads();\n";
-        o << "  if (base == 0 && threadIdx.x == 0) c0[swz(0)] = cx<" << T << ">{1, 0};\n";
+        o << "  if (base != 0) {\n    chunk_copy<" << T << ", false" << ntf << ">(psi, c0, base, rowoff, " << E
+          << ", " << CF << ");\n    return;\n  }\n";
+        o << "  if (threadIdx.x == 0) c0[swz(0)] = cx<" << T << ">{1, 0};\n";
     } else {
         o << "  chunk_copy<" << T << ", true" << ntf << ">(psi, c0, base, rowoff, " << E << ", " << CF << ");\n";
     }
