@@ -557,7 +557,7 @@ def main_batch(args, hip, np):
            "latency_ms_p99_by_n": {str(n): round(float(np.percentile(lats[ns == n], 99)), 4)
                                    for n in sorted(set(ns.tolist()))},
            "gradients": int(info["probes"]), "passes": int(info["passes"]), "workgroups": int(info["workgroups"]),
-           "tuning_K_by_n": tuning["K_by_n"],
+           "tuning": {k: v for k, v in tuning.items() if k != "timings_ms"},
            "sample_vs_single_circuit": {"circuits": len(sample), "max_abs_dE": float(f"{max_de:.3e}"),
                                         "max_abs_dgrad": float(f"{max_dg:.3e}"),
                                         "single_path_s_per_circuit": round(t_single / len(sample), 5)},

@@ -84,6 +84,8 @@ _SIGNATURES = {
     "qsim_batch_create": ([ctypes.POINTER(_P)], _I),
     "qsim_batch_destroy": ([_P], _I),
     "qsim_batch_set_tuning": ([_P, _P, _I, _I], _I),
+    "qsim_batch_set_gang": ([_P, _P, _I], _I),
+    "qsim_batch_set_fusion": ([_P, _P, _I], _I),
     "qsim_batch_load": ([_P, _I, _P, _P, _P, _P, _P, _P, _P, _U64, _P], _I),
     "qsim_batch_run": ([_P, _I, _P, _P, _P, _P], _I),
     "qsim_batch_info": ([_P, _P, _I], _I),

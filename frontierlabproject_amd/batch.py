@@ -4,16 +4,19 @@
 The circuits are packed into flat op lists over one shared complex64 arena
 (fixed gates, their inverses and the rotation generators stored once; each
 rotation angle contributes U(theta) and U(theta)^dag).  The C layer plans every
-circuit into chunk passes and a persistent kernel runs them, one workgroup per
-circuit at a time, largest first; states of at most 12 qubits stay in LDS.  The
-launch is captured once as a hipGraph and replayed.
+circuit into chunk passes and a persistent kernel runs them, largest first,
+each circuit on one workgroup or, above a size threshold, on a gang of
+workgroups; states of at most 12 qubits stay in LDS.  The launch is captured
+once as a hipGraph and replayed.
 
 Gradients are with respect to the rotation angles (RX, RY, RZ) in gate order,
 by the adjoint method of ``adjoint.py`` (probes Im<lam|G|psi>, G = X, Y, Z).
 
-``autotune`` picks the chunk bits K of streamed (> 12-qubit) circuits per
-qubit count by timing candidate values on a sample of the batch, and caches
-the choice in a JSON file so later runs reproduce it.
+``autotune`` picks, per qubit count, the forward-sweep form (fused dense
+steps on the matrix cores or gate-by-gate slot steps) and the chunk bits of
+streamed circuits, then the gang threshold (how many workgroups share each
+large circuit) on the whole batch, and caches the choice in a JSON file so
+later runs reproduce it.
 """
 import ctypes
 import json
@@ -115,7 +118,8 @@ class BatchSimulator:
     """<Z_mask> and the adjoint gradient of every circuit, one persistent launch.
 
     ``zmasks``: per-circuit observable masks (default Z on qubit 0).
-    ``tuning``: {"K_by_n": {n: K}, "blocks_per_cu": int} (see ``autotune``)."""
+    ``tuning``: {"K_by_n": {n: K}, "gang_by_n": {n: G}, "fuse_by_n": {n: 0 or 1},
+    "blocks_per_cu": int} (see ``autotune``)."""
 
     def __init__(self, circuits, zmasks=None, tuning=None, packed=None):
         self.packed = packed if packed is not None else pack(circuits)
@@ -130,6 +134,16 @@ class BatchSimulator:
             for n, k in tuning.get("K_by_n", {}).items():
                 kbn[int(n)] = int(k)
             hip._check(hip.lib.qsim_batch_set_tuning(self._h, kbn.ctypes.data, 41, int(tuning.get("blocks_per_cu", 0))))
+            if "gang_by_n" in tuning:
+                gbn = np.asarray([default_gang(n) for n in range(41)], dtype=np.int32)
+                for n, g in tuning["gang_by_n"].items():
+                    gbn[int(n)] = int(g)
+                hip._check(hip.lib.qsim_batch_set_gang(self._h, gbn.ctypes.data, 41))
+            if "fuse_by_n" in tuning:
+                fbn = np.zeros(41, dtype=np.int32)
+                for n, f in tuning["fuse_by_n"].items():
+                    fbn[int(n)] = int(f)
+                hip._check(hip.lib.qsim_batch_set_fusion(self._h, fbn.ctypes.data, 41))
         hip._check(hip.lib.qsim_batch_load(
             self._h, self.ncirc, p["nq"].ctypes.data, self.zmask.ctypes.data, p["fwd"].ctypes.data,
             p["fwd_off"].ctypes.data, p["rev"].ctypes.data, p["rev_off"].ctypes.data, p["mats"].ctypes.data,
@@ -148,9 +162,9 @@ class BatchSimulator:
         return e, grads, lat, tot.value
 
     def info(self):
-        buf = (ctypes.c_int64 * 5)()
-        hip.lib.qsim_batch_info(self._h, ctypes.addressof(buf), 5)
-        return dict(zip(("circuits", "probes", "passes", "workgroups", "hbm_slot_bytes"), list(buf)))
+        buf = (ctypes.c_int64 * 7)()
+        hip.lib.qsim_batch_info(self._h, ctypes.addressof(buf), 7)
+        return dict(zip(("circuits", "probes", "passes", "workgroups", "hbm_slot_bytes", "tasks", "gangs"), list(buf)))
 
     def close(self):
         if getattr(self, "_h", None):
@@ -164,48 +178,109 @@ class BatchSimulator:
             pass
 
 
+def default_gang(n):
+    """The library's default workgroups per circuit: 2^(n-20) above 20 qubits."""
+    return min(64, 1 << (n - 20)) if n > 20 else 1
+
+
 def default_cache_path():
     return os.environ.get("QSIM_AUTOTUNE_CACHE",
                           os.path.join(os.path.expanduser("~"), ".cache", "qgpusim", "batch_autotune.json"))
 
 
-def autotune(spec, candidates=(10, 11, 12), per_n=6, cache_path=None, log=None):
-    """Chunk bits K per qubit count for streamed circuits of the batch `spec`
-    ([(n, depth, seed)]): for each n > 12 present, a sample of `per_n` of its
-    circuits runs once per candidate K (after one warm-up run) and the fastest
-    K is kept.  Results are cached under the workload signature; a cached
-    entry is returned as is."""
+def gang_policy(threshold, gmax=32):
+    """Workgroups per circuit for a gang threshold T: 2^(n - T) above T qubits
+    (at most gmax), else 1."""
+    return {n: (min(gmax, 1 << (n - threshold)) if n > threshold else 1) for n in range(6, 41)}
+
+
+def _time_batch(packed, tuning, reps=2):
+    """Milliseconds of the launch (best of `reps` after one warm-up run)."""
+    sim = BatchSimulator(None, tuning=tuning, packed=packed)
+    try:
+        sim.run(use_graph=True)
+        return min(sim.run(use_graph=True)[3] for _ in range(reps))
+    finally:
+        sim.close()
+
+
+def autotune(spec, k_candidates=(11, 12), fuse_candidates=(0, 1), thresholds=(16, 17, 18, 19, 20, 23),
+             per_n=48, cache_path=None, log=None):
+    """Tuning for the batch `spec` ([(n, depth, seed)]), in two stages.
+
+    1. Per qubit count n present: the forward-sweep form (fused dense steps on
+       the matrix cores, or gate-by-gate slot steps) and, for streamed circuits
+       (n > 12), the chunk bits K, timed on a sample of up to `per_n` circuits
+       of that size run alone (gangs as the library's default).  K is at most
+       12: two complex64 chunks of 2^12 amplitudes are what the kernel's LDS
+       holds, so larger chunks are not candidates.
+    2. Whole batch: the gang threshold T (circuits above T qubits run on
+       2^(n - T) workgroups, at most 32; T = 23 runs every circuit of this
+       family on one workgroup), timed on the full batch with stage 1's
+       choices.
+
+    Results are cached under the workload signature (qubit counts, depths,
+    seeds and the candidate sets); a cached entry is returned as is."""
     cache_path = cache_path or default_cache_path()
-    ns = sorted({n for n, _, _ in spec if n > 12})
-    key = json.dumps({"ns": ns, "candidates": list(candidates), "per_n": per_n})
+    key = autotune_key(spec, k_candidates, fuse_candidates, thresholds, per_n)
     cache = {}
     if os.path.exists(cache_path):
         with open(cache_path) as f:
             cache = json.load(f)
     if key in cache:
         return cache[key]
-    choice, timings = {}, {}
+    ns = sorted({n for n, _, _ in spec})
+    k_by_n, fuse_by_n, stage1 = {}, {}, {}
     for n in ns:
         sample = [s for s in spec if s[0] == n][:per_n]
-        circuits = random_batch(sample)
-        packed = pack(circuits)
-        best = None
-        timings[n] = {}
-        for K in candidates:
-            sim = BatchSimulator(None, tuning={"K_by_n": {n: K}}, packed=packed)
-            sim.run(use_graph=False)
-            t = min(sim.run(use_graph=False)[3] for _ in range(2))
-            sim.close()
-            timings[n][K] = round(t, 4)
-            if best is None or t < best[1]:
-                best = (K, t)
-        choice[n] = best[0]
+        packed = pack(random_batch(sample))
+        best, stage1[n] = None, {}
+        for K in (k_candidates if n > 12 else (n,)):
+            for fuse in fuse_candidates:
+                tun = {"K_by_n": {n: min(K, 12)}, "fuse_by_n": {n: fuse}}
+                t = _time_batch(packed, tun)
+                stage1[n][f"K{K}_fuse{fuse}"] = round(t, 4)
+                if best is None or t < best[2]:
+                    best = (K, fuse, t)
+        k_by_n[n], fuse_by_n[n] = min(best[0], 12), best[1]
         if log:
-            log(f"autotune n={n}: {timings[n]} -> K={best[0]}")
-    entry = {"K_by_n": {str(n): k for n, k in choice.items()}, "blocks_per_cu": 0,
-             "timings_ms": {str(n): {str(k): v for k, v in t.items()} for n, t in timings.items()}}
+            log(f"autotune n={n}: {stage1[n]} -> K={k_by_n[n]}, fused={fuse_by_n[n]}")
+    packed = pack(random_batch(spec))
+    stage2, best = {}, None
+    for T in thresholds:
+        tun = {"K_by_n": k_by_n, "fuse_by_n": fuse_by_n, "gang_by_n": gang_policy(T)}
+        t = _time_batch(packed, tun, reps=1)
+        stage2[T] = round(t, 2)
+        if best is None or t < best[1]:
+            best = (T, t)
+        if log:
+            log(f"autotune gang threshold {T}: {t:.1f} ms")
+    entry = {"K_by_n": {str(n): k for n, k in k_by_n.items()},
+             "fuse_by_n": {str(n): f for n, f in fuse_by_n.items()},
+             "gang_threshold": best[0],
+             "gang_by_n": {str(n): g for n, g in gang_policy(best[0]).items() if n in ns},
+             "blocks_per_cu": 0,
+             "timings_ms": {"per_n": {str(n): t for n, t in stage1.items()},
+                            "gang_threshold": {str(T): t for T, t in stage2.items()}}}
     cache[key] = entry
     os.makedirs(os.path.dirname(cache_path) or ".", exist_ok=True)
     with open(cache_path, "w") as f:
         json.dump(cache, f, indent=1)
     return entry
+
+
+def autotune_key(spec, k_candidates=(11, 12), fuse_candidates=(0, 1), thresholds=(16, 17, 18, 19, 20, 23),
+                 per_n=48):
+    """The cache key of an autotune call: the workload's digest and the
+    candidate sets."""
+    sig = {"spec_sha": _spec_digest(spec), "k": list(k_candidates), "fuse": list(fuse_candidates),
+           "thresholds": list(thresholds), "per_n": per_n}
+    return json.dumps(sig, sort_keys=True)
+
+
+def _spec_digest(spec):
+    import hashlib
+    h = hashlib.sha256()
+    for n, d, seed in spec:
+        h.update(f"{n},{d},{seed};".encode())
+    return h.hexdigest()[:16]

@@ -373,7 +373,7 @@ int enqueue_passes(qsim_state *st, const Plan &plan, const DescImage &img, char 
             st->counters[5]++;
         } else {
             HIP_TRY(launch_tile_pass<R>(static_cast<cx<R> *>(st->psi), static_cast<cx<R> *>(lam), st->n, hp, dp + p,
-                                        ds, dm, st->d_probe, st->stream));
+                                        ds, dm, st->d_probe, plan.generic, st->stream));
         }
         st->counters[0]++;
         st->counters[3]++;
@@ -393,8 +393,9 @@ int plan_for(qsim_state *st, const qsim_op *ops, int nops, const void *mats, int
     // two-vector (probe) passes hold two chunks in LDS: at most 2^12 complex64
     // amplitudes each
     const int K = max_probes > 1 ? std::min(st->K, st->dtype == QSIM_C64 ? 12 : 11) : st->K;
+    // kernels compiled ahead of time take the generic slot forms
     if (!plan_passes(st->n, pops.data(), nops, mats, (int)amp_bytes(st->dtype), K, st->cmin, max_probes, plan,
-                     err, max_probes == 1 ? st->fuse : 0, in_map, out_map))
+                     err, max_probes == 1 ? st->fuse : 0, in_map, out_map, !st->jit))
         return fail(err);
     return 0;
 }

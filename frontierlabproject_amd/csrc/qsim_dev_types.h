@@ -96,7 +96,14 @@ enum : int32_t {
     S2_CX_FIRST = 6,     // flip second where first = 1
     S2_CX_SECOND = 7,    // flip first where second = 1
     S2_SWAP = 8,         // exchange index 1 and 2
+    S2_CU_FIRST = 9,     // 2x2 U on the second bit where the first is 1    4 cx
+    S2_CU_SECOND = 10,   // 2x2 U on the first bit where the second is 1    4 cx
 };
+// Generic slot forms (plans for the compiled slot-program kernels): 1q slots
+// are S1_GEN, S1_DIAG or probes, pair slots S2_CPHASE, S2_CU_FIRST or
+// S2_CU_SECOND.  Every form is arithmetic on the registers it changes (no
+// register permutations), so the kernels compile without the register-file
+// copies that permutation forms cost where the kinds are only known at run time.
 
 // Device-side records (plain int32 so one upload carries them).
 struct PassDev {
@@ -104,6 +111,7 @@ struct PassDev {
     int32_t step_off, nprobe, probe_off, pad;
     int32_t pos[kMaxK];          // chunk bit k -> state bit position (ascending)
     int32_t comp[kMaxComp];      // block-index bit m -> state bit position
+    int32_t mat_lo, mat_hi;      // the pass's matrix entries: [mat_lo, mat_hi) of the arena
 };
 
 struct StepDev {

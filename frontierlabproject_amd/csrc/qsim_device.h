@@ -30,6 +30,24 @@ __device__ __forceinline__ cx<R> cmad(cx<R> a, cx<R> b, cx<R> c)  // a*b + c
     return {fma(a.re, b.re, fma(-a.im, b.im, c.re)), fma(a.re, b.im, fma(a.im, b.re, c.im))};
 }
 
+// Read-only device data typed in the constant address space: a uniform
+// address is then read with a scalar load (s_load) into scalar registers.
+#define QSIM_CONST __attribute__((address_space(4)))
+template <typename T>
+__device__ __forceinline__ const QSIM_CONST T *as_const(const T *p)
+{
+    return (const QSIM_CONST T *)p;
+}
+
+// Complex element i of a matrix pointer of any address space (members are
+// read one by one: an address-space-qualified struct cannot bind to the
+// generic reference a struct copy takes).
+template <typename P>
+__device__ __forceinline__ auto ldc(P p, int i) -> cx<decltype(+p[0].re)>
+{
+    return {p[i].re, p[i].im};
+}
+
 template <typename R>
 __device__ __forceinline__ void pair_apply(const cx<R> *u, cx<R> &x, cx<R> &y)
 {
@@ -106,6 +124,39 @@ __device__ __forceinline__ double wave_sum(double v)
     return v;
 }
 
+// One DPP move of a double (both halves) with row-local control CTRL; every
+// lane active, every source lane inside its row.
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double v)
+{
+    const uint64_t u = __builtin_bit_cast(uint64_t, v);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)u, CTRL, 0xf, 0xf, false);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)(u >> 32), CTRL, 0xf, 0xf, false);
+    return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+
+__device__ __forceinline__ double readlane_f64(double v, int lane)
+{
+    const uint64_t u = __builtin_bit_cast(uint64_t, v);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)u, lane);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(u >> 32), lane);
+    return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+
+// Sum over the wave's 64 lanes without LDS round trips: within each row of
+// 16 lanes by DPP (exchange with lane ^ 1, then ^ 2, then add the row
+// rotated by 4 and by 8), then the four rows' sums, read from their first
+// lanes, added in row order.  The order of every addition is fixed, so the
+// result is deterministic; it is wave-uniform.  Needs every lane active.
+__device__ __forceinline__ double wave_sum_rows(double v)
+{
+    v += dpp_f64<0xb1>(v);   // quad_perm [1,0,3,2]
+    v += dpp_f64<0x4e>(v);   // quad_perm [2,3,0,1]
+    v += dpp_f64<0x124>(v);  // row_ror:4
+    v += dpp_f64<0x128>(v);  // row_ror:8
+    return ((readlane_f64(v, 0) + readlane_f64(v, 16)) + readlane_f64(v, 32)) + readlane_f64(v, 48);
+}
+
 // A wave-uniform 64-bit value pinned to scalar registers, so that a lane
 // offset added to a pointer plus it becomes the scalar-base + 32-bit-offset
 // form of a global access (no 64-bit address arithmetic per access on the
@@ -128,8 +179,8 @@ __device__ __forceinline__ cx<R> &lds_at(cx<R> *buf, int byte)
 }
 
 // 1q slot arithmetic on register bit B (applied to one vector).
-template <typename R, int B>
-__device__ __forceinline__ void apply1(cx<R> *a, int kind, const cx<R> *__restrict__ mp)
+template <typename R, int B, typename MP>
+__device__ __forceinline__ void apply1(cx<R> *a, int kind, MP mp)
 {
     if (kind & S1_SWAP) {  // U = V X: exchange each pair, then apply V
 #pragma unroll
@@ -143,25 +194,25 @@ __device__ __forceinline__ void apply1(cx<R> *a, int kind, const cx<R> *__restri
     }
     switch (kind) {
     case S1_GEN: {
-        const cx<R> u[4] = {mp[0], mp[1], mp[2], mp[3]};
+        const cx<R> u[4] = {ldc(mp, 0), ldc(mp, 1), ldc(mp, 2), ldc(mp, 3)};
         reg1<R, B>(a, u);
         break;
     }
     case S1_DIAG: {
-        const cx<R> d0 = mp[0], d1 = mp[1];
+        const cx<R> d0 = ldc(mp, 0), d1 = ldc(mp, 1);
 #pragma unroll
         for (int m = 0; m < 16; ++m) a[m] = cmul(a[m], ((m >> B) & 1) ? d1 : d0);
         break;
     }
     case S1_PHASE: {
-        const cx<R> d1 = mp[0];
+        const cx<R> d1 = ldc(mp, 0);
 #pragma unroll
         for (int m = 0; m < 16; ++m)
             if ((m >> B) & 1) a[m] = cmul(a[m], d1);
         break;
     }
     case S1_REAL: {
-        const cx<R> r01 = mp[0], r23 = mp[1];
+        const cx<R> r01 = ldc(mp, 0), r23 = ldc(mp, 1);
 #pragma unroll
         for (int m = 0; m < 16; ++m)
             if (!((m >> B) & 1)) {
@@ -173,7 +224,7 @@ __device__ __forceinline__ void apply1(cx<R> *a, int kind, const cx<R> *__restri
         break;
     }
     case S1_RI: {  // [[p, iq], [ir, t]]
-        const cx<R> pq = mp[0], rt = mp[1];
+        const cx<R> pq = ldc(mp, 0), rt = ldc(mp, 1);
 #pragma unroll
         for (int m = 0; m < 16; ++m)
             if (!((m >> B) & 1)) {
@@ -185,7 +236,7 @@ __device__ __forceinline__ void apply1(cx<R> *a, int kind, const cx<R> *__restri
         break;
     }
     case S1_REAL1: {  // x' = x + a y, y' = b x + y
-        const cx<R> ab = mp[0];
+        const cx<R> ab = ldc(mp, 0);
 #pragma unroll
         for (int m = 0; m < 16; ++m)
             if (!((m >> B) & 1)) {
@@ -197,7 +248,7 @@ __device__ __forceinline__ void apply1(cx<R> *a, int kind, const cx<R> *__restri
         break;
     }
     case S1_RI1: {  // x' = x + i q y, y' = i r x + y
-        const cx<R> qr = mp[0];
+        const cx<R> qr = ldc(mp, 0);
 #pragma unroll
         for (int m = 0; m < 16; ++m)
             if (!((m >> B) & 1)) {
@@ -209,7 +260,7 @@ __device__ __forceinline__ void apply1(cx<R> *a, int kind, const cx<R> *__restri
         break;
     }
     case S1_REALG: {  // x' = x + a y, y' = b x + d y (a, b, d real)
-        const cx<R> ab = mp[0], d0 = mp[1];
+        const cx<R> ab = ldc(mp, 0), d0 = ldc(mp, 1);
 #pragma unroll
         for (int m = 0; m < 16; ++m)
             if (!((m >> B) & 1)) {
@@ -221,7 +272,7 @@ __device__ __forceinline__ void apply1(cx<R> *a, int kind, const cx<R> *__restri
         break;
     }
     case S1_GEN1: {  // x' = x + b y, y' = c x + d y
-        const cx<R> b = mp[0], c = mp[1], d = mp[2];
+        const cx<R> b = ldc(mp, 0), c = ldc(mp, 1), d = ldc(mp, 2);
 #pragma unroll
         for (int m = 0; m < 16; ++m)
             if (!((m >> B) & 1)) {
@@ -287,36 +338,71 @@ __device__ __forceinline__ void apply1(cx<R> *a, int kind, const cx<R> *__restri
     }
 }
 
-// One 1q slot on register bit B.
-template <typename R, int NV, int B>
-__device__ __forceinline__ void slot1(cx<R> *a, cx<R> *l, int kind, const cx<R> *__restrict__ mp, int probe,
-                                      double *pacc, bool act)
+// Generic 1q forms (see S2_CU_FIRST): general 2x2 or diagonal.
+template <typename R, int B, typename MP>
+__device__ __forceinline__ void apply1g(cx<R> *a, int kind, MP mp)
+{
+    if (kind == S1_GEN) {
+        const cx<R> u[4] = {ldc(mp, 0), ldc(mp, 1), ldc(mp, 2), ldc(mp, 3)};
+        reg1<R, B>(a, u);
+    } else if (kind == S1_DIAG) {
+        const cx<R> d0 = ldc(mp, 0), d1 = ldc(mp, 1);
+#pragma unroll
+        for (int m = 0; m < 16; ++m) a[m] = cmul(a[m], ((m >> B) & 1) ? d1 : d0);
+    }
+}
+
+// One 1q slot on register bit B.  u: the slot matrix's first four entries,
+// already read; mp: the matrix itself (a probe's second matrix follows G).
+template <typename R, int NV, int B, bool GENERIC, typename UP, typename MP>
+__device__ __forceinline__ void slot1(cx<R> *a, cx<R> *l, int kind, UP u, MP mp, int probe, double *pacc, bool act)
 {
     if (kind == S1_SKIP) return;
     if (kind == S1_PROBE || kind == S1_PROBE_GEN) {
         if (NV == 2) {
-            const cx<R> g[4] = {mp[0], mp[1], mp[2], mp[3]};
-            const double v = wave_sum(act ? probe1<R, B>(a, l, g) : 0.0);
+            const cx<R> g[4] = {ldc(u, 0), ldc(u, 1), ldc(u, 2), ldc(u, 3)};
+            const double v = wave_sum_rows(act ? probe1<R, B>(a, l, g) : 0.0);
             if ((threadIdx.x & 63) == 0) pacc[probe * kWaves + (threadIdx.x >> 6)] += v;
             if (kind == S1_PROBE_GEN) {
-                const cx<R> u[4] = {mp[4], mp[5], mp[6], mp[7]};
-                reg1<R, B>(a, u);
-                reg1<R, B>(l, u);
+                const cx<R> w[4] = {ldc(mp, 4), ldc(mp, 5), ldc(mp, 6), ldc(mp, 7)};
+                reg1<R, B>(a, w);
+                reg1<R, B>(l, w);
             }
         }
         return;
     }
-    apply1<R, B>(a, kind, mp);
-    if (NV == 2) apply1<R, B>(l, kind, mp);
+    if (GENERIC) {
+        apply1g<R, B>(a, kind, u);
+        if (NV == 2) apply1g<R, B>(l, kind, u);
+    } else {
+        apply1<R, B>(a, kind, u);
+        if (NV == 2) apply1<R, B>(l, kind, u);
+    }
+}
+
+// 2x2 U (row-major) on register bit T over the register groups whose bit C
+// is 1.
+template <typename R, int C, int T, typename MP>
+__device__ __forceinline__ void cu2(cx<R> *a, MP mp)
+{
+    const cx<R> u0 = ldc(mp, 0), u1 = ldc(mp, 1), u2 = ldc(mp, 2), u3 = ldc(mp, 3);
+#pragma unroll
+    for (int m = 0; m < 16; ++m)
+        if (((m >> C) & 1) && !((m >> T) & 1)) {
+            const int m1 = m | (1 << T);
+            const cx<R> x = a[m], y = a[m1];
+            a[m] = cmad(u0, x, cmul(u1, y));
+            a[m1] = cmad(u2, x, cmul(u3, y));
+        }
 }
 
 // pair slot arithmetic on register bits (F, S), F < S, index b_F + 2 b_S.
-template <typename R, int F, int S>
-__device__ __forceinline__ void apply2(cx<R> *a, int kind, const cx<R> *__restrict__ mp)
+template <typename R, int F, int S, typename MP>
+__device__ __forceinline__ void apply2(cx<R> *a, int kind, MP mp)
 {
     switch (kind) {
     case S2_CPHASE: {
-        const cx<R> d3 = mp[0];
+        const cx<R> d3 = ldc(mp, 0);
 #pragma unroll
         for (int m = 0; m < 16; ++m)
             if (((m >> F) & 1) && ((m >> S) & 1)) a[m] = cmul(a[m], d3);
@@ -353,8 +439,30 @@ __device__ __forceinline__ void apply2(cx<R> *a, int kind, const cx<R> *__restri
             }
         break;
     }
+    case S2_CU_FIRST:
+        cu2<R, F, S>(a, mp);
+        break;
+    case S2_CU_SECOND:
+        cu2<R, S, F>(a, mp);
+        break;
     default:
         break;
+    }
+}
+
+// Generic pair forms (see S2_CU_FIRST): controlled phase or controlled 2x2.
+template <typename R, int F, int S, typename MP>
+__device__ __forceinline__ void apply2g(cx<R> *a, int kind, MP mp)
+{
+    if (kind == S2_CPHASE) {
+        const cx<R> d3 = ldc(mp, 0);
+#pragma unroll
+        for (int m = 0; m < 16; ++m)
+            if (((m >> F) & 1) && ((m >> S) & 1)) a[m] = cmul(a[m], d3);
+    } else if (kind == S2_CU_FIRST) {
+        cu2<R, F, S>(a, mp);
+    } else if (kind == S2_CU_SECOND) {
+        cu2<R, S, F>(a, mp);
     }
 }
 
@@ -375,12 +483,17 @@ __device__ __forceinline__ void dense16(cx<R> *a, const cx<R> *__restrict__ M)
 }
 
 // One pair slot on register bits (F, S), F < S.
-template <typename R, int NV, int F, int S>
-__device__ __forceinline__ void slot2(cx<R> *a, cx<R> *l, int kind, const cx<R> *__restrict__ mp)
+template <typename R, int NV, int F, int S, bool GENERIC, typename MP>
+__device__ __forceinline__ void slot2(cx<R> *a, cx<R> *l, int kind, MP mp)
 {
     if (kind == S2_SKIP) return;
-    apply2<R, F, S>(a, kind, mp);
-    if (NV == 2) apply2<R, F, S>(l, kind, mp);
+    if (GENERIC) {
+        apply2g<R, F, S>(a, kind, mp);
+        if (NV == 2) apply2g<R, F, S>(l, kind, mp);
+    } else {
+        apply2<R, F, S>(a, kind, mp);
+        if (NV == 2) apply2<R, F, S>(l, kind, mp);
+    }
 }
 
 // ----------------------------------------------------------------------------
@@ -495,7 +608,8 @@ __device__ __forceinline__ void chunk_copy(cx<R> *__restrict__ g, cx<R> *__restr
     }
 }
 
-__device__ __forceinline__ uint64_t chunk_base(uint64_t c, const PassDev *__restrict__ pd)
+template <typename PP>
+__device__ __forceinline__ uint64_t chunk_base(uint64_t c, PP pd)
 {
     uint64_t base = 0;
     for (int m = 0; m < pd->ncomp; ++m)
@@ -566,18 +680,19 @@ __device__ __forceinline__ void chunk_copy_in_mapped_d(const cx<double> *__restr
 }
 
 // The register steps of one pass over the chunk staged in LDS (c0, and c1 for
-// the second vector).  Ends with a barrier after every step.
-template <typename R, int NV>
-__device__ __forceinline__ void run_steps(cx<R> *c0, cx<R> *c1, const PassDev *__restrict__ pd,
-                                          const StepDev *__restrict__ steps, const cx<R> *__restrict__ mats,
-                                          double *pacc)
+// the second vector).  Ends with a barrier after every step.  The pass, step
+// and matrix pointers may be generic or typed in the constant address space
+// (QSIM_CONST: the descriptors are then read with scalar loads).  GENERIC:
+// the plan uses only the generic slot forms (see S2_CU_FIRST).
+template <typename R, int NV, bool GENERIC, typename PP, typename SP, typename MP>
+__device__ __forceinline__ void run_steps(cx<R> *c0, cx<R> *c1, PP pd, SP steps, MP mats, double *pacc)
 {
     const int E = 1 << pd->K;
     const int ns = pd->nsteps;
-    const StepDev *st = steps + pd->step_off;
+    const SP st = steps + pd->step_off;
     const int ngroups = E >> kG;
     for (int si = 0; si < ns; ++si) {
-        const StepDev *sd = st + si;
+        const SP sd = st + si;
         const int r0 = sd->bits[0], r1 = sd->bits[1], r2 = sd->bits[2], r3 = sd->bits[3];
         const int s0 = sd->sbits[0], s1 = sd->sbits[1], s2 = sd->sbits[2], s3 = sd->sbits[3];
         const int nr = sd->nrounds, dn = sd->dense, sc = sd->scale;
@@ -590,6 +705,14 @@ __device__ __forceinline__ void run_steps(cx<R> *c0, cx<R> *c1, const PassDev *_
 #pragma unroll
         for (int m = 0; m < 16; ++m)
             moff[m] = ((m & 1) ? b0 : 0) ^ ((m & 2) ? b1 : 0) ^ ((m & 4) ? b2 : 0) ^ ((m & 8) ? b3 : 0);
+        if constexpr (GENERIC && NV == 1 && sizeof(R) == 4) {
+            // fused step of a complex64 one-vector pass: the matrix cores
+            if (dn >= 0) {
+                dense16_mfma<kTPB>(c0, mats + dn, moff, s0, s1, s2, s3, ngroups, sc >= 0 ? mats + sc : MP(nullptr));
+                __syncthreads();
+                continue;
+            }
+        }
         // wave-uniform trip count: probes reduce across all 64 lanes
         for (int gb = 0; gb < ngroups; gb += kTPB) {
             const int gi = gb + threadIdx.x;
@@ -605,47 +728,63 @@ __device__ __forceinline__ void run_steps(cx<R> *c0, cx<R> *c1, const PassDev *_
             }
             if (dn >= 0) {
                 // rows go straight back to LDS: no second 16-amplitude array
-                const cx<R> f = sc >= 0 ? mats[sc] : cx<R>{R(1), R(0)};
+                const cx<R> f = sc >= 0 ? ldc(mats, sc) : cx<R>{R(1), R(0)};
 #pragma unroll
                 for (int r = 0; r < 16; ++r) {
-                    const cx<R> *M = mats + dn + 16 * r;
-                    cx<R> acc = cmul(M[0], a[0]);
+                    const MP M = mats + dn + 16 * r;
+                    cx<R> acc = cmul(ldc(M, 0), a[0]);
 #pragma unroll
-                    for (int c = 1; c < 16; ++c) acc = cmad(M[c], a[c], acc);
+                    for (int c = 1; c < 16; ++c) acc = cmad(ldc(M, c), a[c], acc);
                     if (sc >= 0) acc = cmul(acc, f);
                     if (act) c0[sb ^ moff[r]] = acc;
                 }
                 if (NV == 2) {
 #pragma unroll
                     for (int r = 0; r < 16; ++r) {
-                        const cx<R> *M = mats + dn + 16 * r;
-                        cx<R> acc = cmul(M[0], l[0]);
+                        const MP M = mats + dn + 16 * r;
+                        cx<R> acc = cmul(ldc(M, 0), l[0]);
 #pragma unroll
-                        for (int c = 1; c < 16; ++c) acc = cmad(M[c], l[c], acc);
+                        for (int c = 1; c < 16; ++c) acc = cmad(ldc(M, c), l[c], acc);
                         if (sc >= 0) acc = cmul(acc, f);
                         if (act) c1[sb ^ moff[r]] = acc;
                     }
                 }
                 continue;
             } else {
-                // one copy of the round code, run nrounds times
+                // one copy of the round code, run nrounds times.  Every
+                // slot's kind and matrix pointer, and the first four entries
+                // of each 1q slot's matrix, are read before any slot runs, so
+                // the loads are in flight together (matrix arenas end with at
+                // least 256 bytes of padding).
 #pragma unroll 1
                 for (int r = 0; r < nr; ++r) {
                     const int b = r * kSlotsPerRound;
-                    slot1<R, NV, 0>(a, l, sd->kind[b + 0], mats + sd->mat[b + 0], sd->probe[4 * r + 0], pacc, act);
-                    slot1<R, NV, 1>(a, l, sd->kind[b + 1], mats + sd->mat[b + 1], sd->probe[4 * r + 1], pacc, act);
-                    slot1<R, NV, 2>(a, l, sd->kind[b + 2], mats + sd->mat[b + 2], sd->probe[4 * r + 2], pacc, act);
-                    slot1<R, NV, 3>(a, l, sd->kind[b + 3], mats + sd->mat[b + 3], sd->probe[4 * r + 3], pacc, act);
-                    slot2<R, NV, 0, 1>(a, l, sd->kind[b + 4], mats + sd->mat[b + 4]);
-                    slot2<R, NV, 2, 3>(a, l, sd->kind[b + 5], mats + sd->mat[b + 5]);
-                    slot2<R, NV, 0, 2>(a, l, sd->kind[b + 6], mats + sd->mat[b + 6]);
-                    slot2<R, NV, 1, 3>(a, l, sd->kind[b + 7], mats + sd->mat[b + 7]);
-                    slot2<R, NV, 0, 3>(a, l, sd->kind[b + 8], mats + sd->mat[b + 8]);
-                    slot2<R, NV, 1, 2>(a, l, sd->kind[b + 9], mats + sd->mat[b + 9]);
+                    int kd[kSlotsPerRound];
+                    MP mp[kSlotsPerRound];
+                    cx<R> u[4][4];
+#pragma unroll
+                    for (int j = 0; j < kSlotsPerRound; ++j) {
+                        kd[j] = sd->kind[b + j];
+                        mp[j] = mats + sd->mat[b + j];
+                    }
+#pragma unroll
+                    for (int j = 0; j < 4; ++j)
+#pragma unroll
+                        for (int i = 0; i < 4; ++i) u[j][i] = ldc(mp[j], i);
+                    slot1<R, NV, 0, GENERIC>(a, l, kd[0], u[0], mp[0], sd->probe[4 * r + 0], pacc, act);
+                    slot1<R, NV, 1, GENERIC>(a, l, kd[1], u[1], mp[1], sd->probe[4 * r + 1], pacc, act);
+                    slot1<R, NV, 2, GENERIC>(a, l, kd[2], u[2], mp[2], sd->probe[4 * r + 2], pacc, act);
+                    slot1<R, NV, 3, GENERIC>(a, l, kd[3], u[3], mp[3], sd->probe[4 * r + 3], pacc, act);
+                    slot2<R, NV, 0, 1, GENERIC>(a, l, kd[4], mp[4]);
+                    slot2<R, NV, 2, 3, GENERIC>(a, l, kd[5], mp[5]);
+                    slot2<R, NV, 0, 2, GENERIC>(a, l, kd[6], mp[6]);
+                    slot2<R, NV, 1, 3, GENERIC>(a, l, kd[7], mp[7]);
+                    slot2<R, NV, 0, 3, GENERIC>(a, l, kd[8], mp[8]);
+                    slot2<R, NV, 1, 2, GENERIC>(a, l, kd[9], mp[9]);
                 }
             }
             if (sc >= 0) {  // scalar factored out of earlier gates
-                const cx<R> f = mats[sc];
+                const cx<R> f = ldc(mats, sc);
 #pragma unroll
                 for (int m = 0; m < 16; ++m) {
                     a[m] = cmul(a[m], f);
@@ -677,21 +816,21 @@ __device__ __forceinline__ void run_steps(cx<R> *c0, cx<R> *c1, const PassDev *_
 // read and written by one lane set of one wave, in program order.
 // moff[m]: LDS slot offset of register index m; s0..s3: the step's register
 // bits ascending; ng: register groups in the chunk; sc: optional factor.
-template <int TPB = kTPB>
-__device__ __forceinline__ void dense16_mfma(cx<float> *c0, const cx<float> *__restrict__ M, const int *moff,
-                                             int s0, int s1, int s2, int s3, int ng, const cx<float> *sc)
+template <int TPB = kTPB, typename MP = const cx<float> *>
+__device__ __forceinline__ void dense16_mfma(cx<float> *c0, MP M, const int *moff, int s0, int s1, int s2, int s3,
+                                             int ng, MP sc)
 {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int j = lane & 15, q = lane >> 4;
     float ar[4], ai[4], as[4];
 #pragma unroll
     for (int kb = 0; kb < 4; ++kb) {
-        const cx<float> m = M[16 * j + 4 * kb + q];
+        const cx<float> m = ldc(M, 16 * j + 4 * kb + q);
         ar[kb] = m.re;
         ai[kb] = m.im;
         as[kb] = m.re + m.im;
     }
-    const cx<float> f = sc ? *sc : cx<float>{1.f, 0.f};
+    const cx<float> f = sc ? ldc(sc, 0) : cx<float>{1.f, 0.f};
     for (int gb = wave * 64; gb < ng; gb += TPB) {  // wave-uniform trip count
 #pragma unroll
         for (int nb = 0; nb < 4; ++nb) {

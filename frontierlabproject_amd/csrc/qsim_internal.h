@@ -57,7 +57,7 @@ hipError_t launch_permute(const cx<R> *in, cx<R> *out, const PermArgs &a, hipStr
 template <typename R>
 hipError_t launch_tile_pass(cx<R> *psi, cx<R> *lam, int n, const PassDev &host_pass,
                             const PassDev *d_pass, const StepDev *d_steps, const cx<R> *d_mats,
-                            double *d_probe_partial, hipStream_t s);
+                            double *d_probe_partial, bool generic, hipStream_t s);
 hipError_t launch_probe_reduce(const double *partial, int nblocks, int nprobe,
                                const int32_t *d_ids, double *out, hipStream_t s);
 template <typename R>

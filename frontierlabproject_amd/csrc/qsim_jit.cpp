@@ -57,7 +57,8 @@ void emit_step(std::ostringstream &o, const StepDev &sd, int E, const char *T, i
         for (int m = 0; m < 16; ++m) o << moff[m] << (m < 15 ? ", " : "");
         o << "};\n    dense16_mfma<" << tpb << ">(c0, mats + " << sd.dense << ", moff, " << sd.sbits[0] << ", " << sd.sbits[1]
           << ", " << sd.sbits[2] << ", " << sd.sbits[3] << ", " << NG << ", "
-          << (sd.scale >= 0 ? "mats + " + std::to_string(sd.scale) : std::string("nullptr")) << ");\n  }\n";
+          << (sd.scale >= 0 ? "mats + " + std::to_string(sd.scale) : std::string("(const cx<float> *)nullptr"))
+          << ");\n  }\n";
         o << "  __syncthreads();\n";
         return;
     }
@@ -110,7 +111,7 @@ void emit_step(std::ostringstream &o, const StepDev &sd, int E, const char *T, i
                     // Im<lam|G|psi> over this step's pairs, one partial per wave
                     o << "    { const cx<" << T << "> g[4] = {mats[" << mo << "], mats[" << mo + 1 << "], mats["
                       << mo + 2 << "], mats[" << mo + 3 << "]};\n";
-                    o << "      const double pv = wave_sum(" << (guard ? "act ? " : "") << "probe1<" << T << ", "
+                    o << "      const double pv = wave_sum_rows(" << (guard ? "act ? " : "") << "probe1<" << T << ", "
                       << j << ">(a, l, g)" << (guard ? " : 0.0" : "") << ");\n";
                     o << "      if ((threadIdx.x & 63) == 0) pacc[" << sd.probe[4 * r + j]
                       << " * kWaves + (threadIdx.x >> 6)] += pv; }\n";

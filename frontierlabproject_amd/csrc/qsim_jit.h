@@ -2,11 +2,13 @@
 //
 // The compiled kernel k_tile_pass reads each step's slot program from device
 // memory and branches on every slot's gate kind, so each slot ends in a join
-// where the 16 amplitudes are copied back into fixed registers.  A specialised
-// kernel has the pass's register bits, LDS offsets and gate kinds as
-// constants: the slots become straight-line code, and register exchanges (X,
-// CX, SWAP) become renames.  Arithmetic and order are those of k_tile_pass;
-// both include qsim_device.h.
+// where the 16 amplitudes are copied back into fixed registers; it runs plans
+// in the generic slot forms, which keep those copies to one per changed
+// amplitude.  A specialised kernel has the pass's register bits, LDS offsets
+// and gate kinds as constants: the slots become straight-line code, register
+// exchanges (X, CX, SWAP) become renames, and the plan uses the cheapest form
+// of each gate.  Both apply the same gates in the same order and include
+// qsim_device.h.
 #pragma once
 #include <hip/hip_runtime.h>
 

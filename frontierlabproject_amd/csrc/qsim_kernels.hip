@@ -402,7 +402,7 @@ hipError_t launch_permute(const cx<R> *in, cx<R> *out, const PermArgs &P, hipStr
 
 // One chunk per block.  complex64 forward passes ask for 4 waves per SIMD
 // (at most 128 VGPRs): the copy batches otherwise push occupancy to 3.
-template <typename R, int NV>
+template <typename R, int NV, bool GENERIC>
 __global__ __launch_bounds__(kTPB)
 __attribute__((amdgpu_waves_per_eu((NV == 1 && sizeof(R) == 4) ? 4 : 1))) void k_tile_pass(
     cx<R> *__restrict__ psi, cx<R> *__restrict__ lam, const PassDev *__restrict__ pd,
@@ -435,7 +435,7 @@ __attribute__((amdgpu_waves_per_eu((NV == 1 && sizeof(R) == 4) ? 4 : 1))) void k
     if (NV == 2)
         for (int t = threadIdx.x; t < nprobe * kWaves; t += kTPB) pacc[t] = 0.0;
     __syncthreads();
-    run_steps<R, NV>(c0, c1, pd, steps, mats, pacc);
+    run_steps<R, NV, GENERIC>(c0, c1, pd, steps, mats, pacc);
     chunk_copy<R, false, true>(psi, c0, base, rowoff, E, cf);
     if (NV == 2) {
         chunk_copy<R, false, true>(lam, c1, base, rowoff, E, cf);
@@ -447,7 +447,7 @@ __attribute__((amdgpu_waves_per_eu((NV == 1 && sizeof(R) == 4) ? 4 : 1))) void k
     }
 }
 
-template <typename R, int NV>
+template <typename R, int NV, bool GENERIC>
 static hipError_t tile_pass_launch(cx<R> *psi, cx<R> *lam, int n, const PassDev &hp, const PassDev *d_pass,
                                    const StepDev *d_steps, const cx<R> *d_mats, double *d_probe_partial,
                                    hipStream_t s)
@@ -455,17 +455,22 @@ static hipError_t tile_pass_launch(cx<R> *psi, cx<R> *lam, int n, const PassDev 
     const size_t sm = (size_t)NV * (sizeof(cx<R>) << hp.K) + 8u * (1u << (hp.K - hp.cfast)) +
                       (NV == 2 ? 8u * kWaves * (size_t)hp.nprobe : 0u);
     const uint64_t blocks = 1ull << (n - hp.K);
-    hipLaunchKernelGGL((k_tile_pass<R, NV>), dim3((unsigned)blocks), dim3(kTPB), sm, s, psi, lam, d_pass, d_steps,
+    hipLaunchKernelGGL((k_tile_pass<R, NV, GENERIC>), dim3((unsigned)blocks), dim3(kTPB), sm, s, psi, lam, d_pass, d_steps,
                        d_mats, d_probe_partial);
     return hipGetLastError();
 }
 
 template <typename R>
 hipError_t launch_tile_pass(cx<R> *psi, cx<R> *lam, int n, const PassDev &hp, const PassDev *d_pass,
-                            const StepDev *d_steps, const cx<R> *d_mats, double *d_probe_partial, hipStream_t s)
+                            const StepDev *d_steps, const cx<R> *d_mats, double *d_probe_partial, bool generic,
+                            hipStream_t s)
 {
-    if (lam) return tile_pass_launch<R, 2>(psi, lam, n, hp, d_pass, d_steps, d_mats, d_probe_partial, s);
-    return tile_pass_launch<R, 1>(psi, nullptr, n, hp, d_pass, d_steps, d_mats, nullptr, s);
+    if (generic) {
+        if (lam) return tile_pass_launch<R, 2, true>(psi, lam, n, hp, d_pass, d_steps, d_mats, d_probe_partial, s);
+        return tile_pass_launch<R, 1, true>(psi, nullptr, n, hp, d_pass, d_steps, d_mats, nullptr, s);
+    }
+    if (lam) return tile_pass_launch<R, 2, false>(psi, lam, n, hp, d_pass, d_steps, d_mats, d_probe_partial, s);
+    return tile_pass_launch<R, 1, false>(psi, nullptr, n, hp, d_pass, d_steps, d_mats, nullptr, s);
 }
 
 // ----------------------------------------------------------------------------
@@ -673,8 +678,14 @@ hipError_t prepare_kernels()
     e = hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
     if (e != hipSuccess) return e;
     const void *fns[] = {
-        reinterpret_cast<const void *>(k_tile_pass<float, 1>), reinterpret_cast<const void *>(k_tile_pass<float, 2>),
-        reinterpret_cast<const void *>(k_tile_pass<double, 1>), reinterpret_cast<const void *>(k_tile_pass<double, 2>),
+        reinterpret_cast<const void *>(k_tile_pass<float, 1, false>),
+        reinterpret_cast<const void *>(k_tile_pass<float, 2, false>),
+        reinterpret_cast<const void *>(k_tile_pass<double, 1, false>),
+        reinterpret_cast<const void *>(k_tile_pass<double, 2, false>),
+        reinterpret_cast<const void *>(k_tile_pass<float, 1, true>),
+        reinterpret_cast<const void *>(k_tile_pass<float, 2, true>),
+        reinterpret_cast<const void *>(k_tile_pass<double, 1, true>),
+        reinterpret_cast<const void *>(k_tile_pass<double, 2, true>),
         reinterpret_cast<const void *>(k_permute<float, true, kTPB>),
         reinterpret_cast<const void *>(k_permute<double, true, kTPB>),
         reinterpret_cast<const void *>(k_pauli_chunk<float>), reinterpret_cast<const void *>(k_pauli_chunk<double>),
@@ -690,7 +701,7 @@ hipError_t prepare_kernels()
     template hipError_t launch_diag1q<R>(cx<R> *, int, int, cx<R>, cx<R>, hipStream_t);                     \
     template hipError_t launch_permute<R>(const cx<R> *, cx<R> *, const PermArgs &, hipStream_t);          \
     template hipError_t launch_tile_pass<R>(cx<R> *, cx<R> *, int, const PassDev &, const PassDev *,        \
-                                            const StepDev *, const cx<R> *, double *, hipStream_t);         \
+                                            const StepDev *, const cx<R> *, double *, bool, hipStream_t);   \
     template hipError_t launch_pauli_chunk<R>(const cx<R> *, cx<R> *, int, const PassDev &,                 \
                                               const PassDev *, const PauliDev *, int, int, double *,        \
                                               hipStream_t);                                                 \

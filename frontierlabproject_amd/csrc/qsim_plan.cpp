@@ -242,6 +242,7 @@ struct Ctx {
     int max_probes;
     const int *loc;      // state bit -> chunk-local bit
     bool real_scale;     // non-unit scalars may be factored out (no probes)
+    bool generic;        // generic slot forms only (S1_GEN, S1_DIAG; S2_CPHASE, S2_CU_*)
 };
 
 int push_mats(Plan &p, const Cd *v, int count)
@@ -268,6 +269,19 @@ inline bool factorable(const Cd *u)
 void classify_1q(Ctx &c, StepBuilder &sb, const Cd *u, int j)
 {
     Plan &p = *c.plan;
+    if (c.generic) {  // diagonal or general, the matrix as given
+        if (is0(u[1]) && is0(u[2])) {
+            const Cd d[2] = {u[0], u[3]};
+            sb.sd.kind[j] = S1_DIAG;
+            sb.sd.mat[j] = push_mats(p, d, 2);
+            sb.cost += 4.0;
+        } else {
+            sb.sd.kind[j] = S1_GEN;
+            sb.sd.mat[j] = push_mats(p, u, 4);
+            sb.cost += 8.0;
+        }
+        return;
+    }
     if (is0(u[1]) && is0(u[2]) &&
         (c.real_scale ? !is0(u[0]) : std::fabs(u[0].re * u[0].re + u[0].im * u[0].im - 1.0) < 1e-12)) {
         // diag(u0, u3) = u0 * diag(1, u3/u0): the phase u0 is applied later
@@ -394,7 +408,7 @@ int place(Ctx &c, StepBuilder &sb, int idx, int rx, int ry)
             // (columns exchanged), whose u0 is: the slot exchanges the pair
             // (a register rename in specialised kernels) and runs V's form.
             const Cd ux[4] = {u[1], u[0], u[3], u[2]};
-            const bool swap = c.real_scale && !(is0(u[1]) && is0(u[2])) && !factorable(u) && factorable(ux) &&
+            const bool swap = !c.generic && c.real_scale && !(is0(u[1]) && is0(u[2])) && !factorable(u) && factorable(ux) &&
                               !(is0(u[0]) && is0(u[3]) && ((is1(u[1]) && is1(u[2])) ||
                                                             (iseq(u[1], 0, -1) && iseq(u[2], 0, 1))));
             classify_1q(c, sb, swap ? ux : u, j);
@@ -423,8 +437,26 @@ int place(Ctx &c, StepBuilder &sb, int idx, int rx, int ry)
         }
         return true;
     };
+    // controlled 2x2: identity on the indices where the control bit is 0,
+    // U on the other two, nothing between the halves
+    auto controlled = [&](int c0, int c1, int t0, int t1, Cd *u) {
+        const int idx[4] = {c0, c1, t0, t1};
+        for (int r = 0; r < 4; ++r)
+            for (int q = 0; q < 4; ++q) {
+                const Cd v = m[4 * idx[r] + idx[q]];
+                if (r < 2 || q < 2) {
+                    if (r == q ? !is1(v) : !is0(v)) return false;
+                }
+            }
+        u[0] = m[4 * t0 + t0], u[1] = m[4 * t0 + t1], u[2] = m[4 * t1 + t0], u[3] = m[4 * t1 + t1];
+        return true;
+    };
     int kind;
+    Cd cu[4];
     if (only({0, 5, 10}, 15)) kind = S2_CPHASE;                  // diag(1,1,1,d)
+    else if (c.generic && controlled(0, 2, 1, 3, cu)) kind = S2_CU_FIRST;   // U on indices 1, 3
+    else if (c.generic && controlled(0, 1, 2, 3, cu)) kind = S2_CU_SECOND;  // U on indices 2, 3
+    else if (c.generic) return 2;                                  // needs a dense step
     else if (only({0, 10, 7, 13}, -1)) kind = S2_CX_FIRST;       // rows 1<->3
     else if (only({0, 5, 11, 14}, -1)) kind = S2_CX_SECOND;      // rows 2<->3
     else if (only({0, 6, 9, 15}, -1)) kind = S2_SWAP;            // rows 1<->2
@@ -433,7 +465,8 @@ int place(Ctx &c, StepBuilder &sb, int idx, int rx, int ry)
     if (j < 0) return 0;
     sb.sd.kind[j] = kind;
     if (kind == S2_CPHASE) sb.sd.mat[j] = push_mats(p, &m[15], 1);
-    sb.cost += kind == S2_CPHASE ? 1.0 : 0.5;
+    if (kind == S2_CU_FIRST || kind == S2_CU_SECOND) sb.sd.mat[j] = push_mats(p, cu, 4);
+    sb.cost += kind == S2_CPHASE ? 1.0 : (kind == S2_CU_FIRST || kind == S2_CU_SECOND) ? 4.0 : 0.5;
     sb.last[rx] = sb.last[ry] = j;
     sb.placed.push_back({2 * j, idx});
     return 1;
@@ -507,9 +540,10 @@ void to_dense(Ctx &c, StepBuilder &sb)
 
 bool plan_passes(int n, const PlanOp *ops, int nops, const void *mats, int elem_bytes, int K, int cmin,
                  int max_probes, Plan &out, std::string &err, int fuse, const int32_t *in_map,
-                 const int32_t *out_map)
+                 const int32_t *out_map, bool generic)
 {
     out = Plan();
+    out.generic = generic;
     if (n < kG || n > 63) {
         err = "tile plan: state must have between 4 and 63 qubits";
         return false;
@@ -621,7 +655,7 @@ bool plan_passes(int n, const PlanOp *ops, int nops, const void *mats, int elem_
         pd.step_off = (int32_t)out.steps.size();
         pd.probe_off = (int32_t)out.probe_ids.size();
         pd.nprobe = 0;
-        Ctx ctx{ops, mats, elem_bytes, &out, &pd, max_probes, loc, !any_probe};
+        Ctx ctx{ops, mats, elem_bytes, &out, &pd, max_probes, loc, !any_probe && !generic, generic};
 
         // ---- register steps ----
         std::vector<int> srem = chosen;
@@ -687,7 +721,7 @@ bool plan_passes(int n, const PlanOp *ops, int nops, const void *mats, int elem_
                 }
             }
             // a slot step whose arithmetic exceeds one dense product becomes dense
-            if (!sb.dense && !sb.has_probe && sb.cost > (fuse == 2 ? 32.0 : 64.0)) to_dense(ctx, sb);
+            if (!generic && !sb.dense && !sb.has_probe && sb.cost > (fuse == 2 ? 32.0 : 64.0)) to_dense(ctx, sb);
             // fill unused register bits with the lowest free chunk bits
             for (int b = 0; b < K && sb.nreg < kG; ++b)
                 if (sb.reg_of(b) < 0) sb.regbit[sb.nreg++] = b;
@@ -734,6 +768,23 @@ bool plan_passes(int n, const PlanOp *ops, int nops, const void *mats, int elem_
         } else {
             last.scale = push_mats(out, &pending, 1);
         }
+    }
+    // each pass's matrix entries, as one range of the arena
+    for (PassDev &pd : out.passes) {
+        int lo = INT32_MAX, hi = 0;
+        auto cover = [&](int off, int count) {
+            lo = std::min(lo, off);
+            hi = std::max(hi, off + count);
+        };
+        for (int si = pd.step_off; si < pd.step_off + pd.nsteps; ++si) {
+            const StepDev &sd = out.steps[si];
+            if (sd.dense >= 0) cover(sd.dense, 256);
+            if (sd.scale >= 0) cover(sd.scale, 1);
+            for (int k = 0; k < kSlots; ++k)
+                if (sd.kind[k] != S1_SKIP) cover(sd.mat[k], k % kSlotsPerRound < 4 ? 8 : 4);
+        }
+        pd.mat_lo = lo == INT32_MAX ? 0 : lo;
+        pd.mat_hi = lo == INT32_MAX ? 0 : std::min(hi, (int)(out.mats.size() / 2));
     }
     return true;
 }

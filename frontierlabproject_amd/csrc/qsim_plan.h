@@ -61,6 +61,9 @@ struct Plan {
     // When set (the input is |0...0>, not yet written): the first pass
     // generates its chunks instead of reading them.
     bool zero_input = false;
+    // Only the generic slot forms (see S2_CU_FIRST): the plan runs on the
+    // compiled slot-program kernels.
+    bool generic = false;
 };
 
 // Scalar factors (a diagonal gate's first entry, H's 1/sqrt2) are taken out of
@@ -83,9 +86,12 @@ struct Plan {
 // input bit positions, so it reads runs of 2^cmin contiguous amplitudes.
 // out_map (n entries, or null): the output goes to another layout (see
 // Plan::out_map), folded into the last pass when it has room.
+// generic: only the generic slot forms, no scalars taken out, and no step
+// made dense for its cost (plans for the compiled slot-program kernels, where
+// the kinds are only known at run time and code size is what limits speed).
 bool plan_passes(int n, const PlanOp *ops, int nops, const void *mats, int elem_bytes, int K,
                  int cmin, int max_probes, Plan &out, std::string &err, int fuse = 0,
-                 const int32_t *in_map = nullptr, const int32_t *out_map = nullptr);
+                 const int32_t *in_map = nullptr, const int32_t *out_map = nullptr, bool generic = false);
 
 // Pauli chunk passes: each term's X support must lie inside its pass's chunk.
 struct PauliPass {

@@ -186,8 +186,10 @@ int qsim_set_tile_params(qsim_state *st, int K, int cmin);
  * forward chunk passes run kernels specialised per pass, with the pass's
  * register bits, LDS offsets and gate kinds compiled in by hiprtc on first
  * use and cached per device for the life of the process.  0: every pass runs
- * the compiled slot-program kernel.  Both apply the same arithmetic in the
- * same order. */
+ * the compiled slot-program kernel, on a plan whose gates take generic
+ * arithmetic forms (general or diagonal 2x2, controlled phase or controlled
+ * 2x2).  Both apply the same gates in the same order; results agree to
+ * rounding. */
 int qsim_set_jit(qsim_state *st, int enable);
 
 /* Host only, no reference counterpart: plan `ops` as qsim_apply_ops would (as
@@ -307,6 +309,22 @@ int qsim_batch_destroy(qsim_batch *b);
  * next qsim_batch_load. */
 int qsim_batch_set_tuning(qsim_batch *b, const int32_t *K_by_n, int nk, int blocks_per_cu);
 
+/* Workgroups per streamed circuit by qubit count (gang_by_n[n], n < nk, each
+ * in [1, 64]; default 2^(n-20) above 20 qubits, else 1): a circuit run by a
+ * gang of G workgroups has each rank take a contiguous 1/G of every pass's
+ * chunks, with a device-wide barrier between passes; results are those of
+ * one workgroup up to the order of the FP64 partial sums (per rank, then
+ * added in rank order).  States held in LDS always use one.  The
+ * launch caps G at a quarter of its workgroups.  Applies to the next
+ * qsim_batch_load. */
+int qsim_batch_set_gang(qsim_batch *b, const int32_t *gang_by_n, int nk);
+/* Forward-sweep form by qubit count (fuse_by_n[n], n < nk, each 0 or 1;
+ * default 1): 1 fuses every register step of the forward sweep into one
+ * 16x16 unitary (the complex128 product of its gates, rounded once) applied
+ * as a complex GEMM on the FP32 matrix cores; 0 runs each gate by its own
+ * arithmetic.  The reverse sweep (gradient probes between its gates) always
+ * runs gate by gate.  Applies to the next qsim_batch_load. */
+int qsim_batch_set_fusion(qsim_batch *b, const int32_t *fuse_by_n, int nk);
 /* Circuit i has num_qubits[i] (6..34) qubits, observable Z on zmask[i],
  * forward ops fwd_ops[fwd_off[i] .. fwd_off[i+1]), reverse ops (QSIM_OP_PROBE
  * allowed, q1 = probe slot < nprobes[i]) rev_ops[rev_off[i] .. rev_off[i+1]);
@@ -324,7 +342,8 @@ int qsim_batch_load(qsim_batch *b, int ncirc, const int32_t *num_qubits, const u
 int qsim_batch_run(qsim_batch *b, int use_graph, double *expect_out, double *probes_out,
                    double *exec_ms_out, double *total_ms_out);
 
-/* [0] circuits, [1] probes, [2] passes, [3] workgroups, [4] HBM slot bytes.
+/* [0] circuits, [1] probes, [2] passes, [3] workgroups, [4] HBM slot bytes,
+ * [5] queue tasks (one per circuit, or per rank of a gang), [6] gangs.
  * Returns the number written. */
 int qsim_batch_info(const qsim_batch *b, int64_t *out, int n);
 
