@@ -7,10 +7,12 @@ Differences from the reference, each deliberate:
 
 * the gate-matrix cache is keyed on (name, params, dtype); the reference's key
   omits the dtype (backend.py:30-35);
-* three-qubit gates (ccx, ccz, cswap) run as their exact 1q/2q networks
-  inside their tile (all three qubits are local there); other ops on three or
-  more qubits raise ``ValueError``; the reference drops both silently
-  (backend.py:448-451);
+* three-qubit gates with exact 1q/2q networks (ccx, ccz, cswap) run as those
+  networks inside their tile (all their qubits are local there); any other
+  op on three or four qubits runs as one dense 16x16 register step in the
+  chunked modes (one 3- or 4-qubit stream kernel in the sequential mode);
+  ops on more qubits raise ``ValueError``; the reference drops them all
+  silently (backend.py:448-451);
 * fused matrices are multiplied in complex128 and then cast;
 * ``qc.global_phase`` is applied (the reference never reads it): folded into
   the matrix of the first op the circuit applies, so it costs no pass;
@@ -48,6 +50,12 @@ _TILE_MODE_CODES = {"cooperative": COOPERATIVE, "graphed": GRAPHED, "sequential"
 USE_NATIVE = True
 
 MAX_CACHE_SIZE = 512
+# Periodic renormalisation (complex64 runs): every RENORM_EVERY tiles with
+# ops, the state's norm is reduced in FP64 on the GPU (one read of the state);
+# when |<psi|psi> - 1| exceeds RENORM_TOL, 1/||psi|| is folded into the first
+# matrix of the next tile with ops, so the scaling costs no pass.
+RENORM_EVERY = 16
+RENORM_TOL = 1e-6
 GATE_CACHE: "OrderedDict" = OrderedDict()
 KERNEL_LAUNCH_OVERHEAD_US = 10.0
 
@@ -105,10 +113,8 @@ def fuse_tile_ops(tile_ops, dtype=np.complex128):
     result = []
     last_gate_idx = {}
 
-    def touches(o, qs):
-        if o[0] == "1q":
-            return o[1] in qs
-        return o[1] in qs or o[2] in qs
+    def touches(o, qs):  # o = (kind, qubits..., U)
+        return any(q in qs for q in o[1:-1])
 
     for op in tile_ops:
         if op[0] == "1q":
@@ -148,14 +154,14 @@ def fuse_tile_ops(tile_ops, dtype=np.complex128):
     return result
 
 
-def fold_global_phase(tile_ops, phase, dtype):
-    """``tile_ops`` with exp(i * phase) folded into the matrix of its first op
-    (product in complex128, then cast); unchanged when phase is 0 or the list
-    is empty."""
-    if not phase or not tile_ops:
+def fold_global_phase(tile_ops, phase, dtype, scale=1.0):
+    """``tile_ops`` with scale * exp(i * phase) folded into the matrix of its
+    first op (product in complex128, then cast); unchanged when the factor is
+    1 or the list is empty."""
+    if (not phase and scale == 1.0) or not tile_ops:
         return tile_ops
     first = tile_ops[0]
-    U = (np.asarray(first[-1], dtype=np.complex128) * np.exp(1j * phase)).astype(dtype)
+    U = (np.asarray(first[-1], dtype=np.complex128) * (scale * np.exp(1j * phase))).astype(dtype)
     return [tuple(first[:-1]) + (U,)] + list(tile_ops[1:])
 
 
@@ -185,15 +191,19 @@ def build_tile_ops(tile: Tile, qubit_map, np_dtype):
             for g, qs in decompose(op.name, local):
                 Ug = get_gate_matrix_host(g, np_dtype)
                 ops.append(("1q", qs[0], Ug) if len(qs) == 1 else ("2q", qs[0], qs[1], Ug))
+        elif len(local) in (3, 4):
+            # one dense register step inside the tile (all its qubits are local)
+            ops.append((f"{len(local)}q", *local, U))
         else:
             raise ValueError(f"gate {op.name!r} acts on {len(local)} qubits; "
-                             "the tile path supports 1- and 2-qubit gates and ccx, ccz, cswap")
+                             "the tile path supports gates on at most 4 qubits")
     return ops, ngates
 
 
 def run_custom_backend(qc, tile_plan: List[Tile], task: str, shots: int = None,
                        use_float32: bool = False, use_cuda_graphs: bool = True,
-                       tile_mode: str = "cooperative", enable_fusion: bool = True, sample_seed: int = 0):
+                       tile_mode: str = "cooperative", enable_fusion: bool = True, sample_seed: int = 0,
+                       renormalize_every: int = None, renormalize_tol: float = RENORM_TOL):
     """Simulate ``qc`` along ``tile_plan`` on the GPU (backend.py:336-522).
 
     Returns ``(statevector, metrics)``; ``metrics`` has the reference's keys:
@@ -207,6 +217,10 @@ def run_custom_backend(qc, tile_plan: List[Tile], task: str, shots: int = None,
     t_sync_s.  With ``task="sampling"`` and ``shots``, ``metrics["counts"]``
     holds outcomes sampled on the GPU from the final state (seed
     ``sample_seed``) over the circuit's measurements, or all qubits.
+    ``renormalize_every`` (default RENORM_EVERY for complex64, 0 = never for
+    complex128): tiles with ops between FP64 norm checks; a norm off by more
+    than ``renormalize_tol`` is corrected in the next tile's first matrix
+    (``metrics["renorm_checks"]``, ``metrics["renormalizations"]``).
     """
     if tile_mode not in ("cooperative", "graphed", "sequential"):
         raise ValueError(f"unknown tile_mode {tile_mode!r}")
@@ -223,6 +237,9 @@ def run_custom_backend(qc, tile_plan: List[Tile], task: str, shots: int = None,
     t_perm = t_apply = 0.0
 
     phase = _global_phase(qc)  # applied with the first op (or to the result when nothing runs)
+    if renormalize_every is None:
+        renormalize_every = RENORM_EVERY if use_float32 else 0
+    scale, since_check, renorm_checks, renorms = 1.0, 0, 0, 0
     sv = qgpusim_cuda.StatevectorF32(num_qubits) if use_float32 else qgpusim_cuda.Statevector(num_qubits)
     identity = list(range(num_qubits))
     t0 = time.perf_counter()
@@ -248,7 +265,7 @@ def run_custom_backend(qc, tile_plan: List[Tile], task: str, shots: int = None,
         if tile_ops:
             if enable_fusion:
                 tile_ops = fuse_tile_ops(tile_ops, np_dtype)
-            tile_ops, phase = fold_global_phase(tile_ops, phase, np_dtype), 0.0
+            tile_ops, phase, scale = fold_global_phase(tile_ops, phase, np_dtype, scale), 0.0, 1.0
             num_fused_ops += len(tile_ops)
             if i == last_ops and layout != identity:
                 sv.apply_tile_permute(mode_code, tile_ops, layout, identity)
@@ -264,6 +281,14 @@ def run_custom_backend(qc, tile_plan: List[Tile], task: str, shots: int = None,
                 num_graph_replays += 1
             else:
                 num_kernel_launches += len(tile_ops)
+            since_check += 1
+            if renormalize_every and since_check >= renormalize_every and i < last_ops:
+                since_check = 0
+                renorm_checks += 1
+                n2 = sv.norm_squared()
+                if abs(n2 - 1.0) > renormalize_tol:
+                    scale = 1.0 / np.sqrt(n2)
+                    renorms += 1
         t_apply += time.perf_counter() - t0
 
     if layout != identity:  # no final all-qubit tile: back to the circuit's qubit order
@@ -316,6 +341,8 @@ def run_custom_backend(qc, tile_plan: List[Tile], task: str, shots: int = None,
         "device_chunk_passes": counters["chunk_passes"],
         "device_folded_permutations": counters["folded_permutations"],
         "device_permutation_passes": counters["permutations"],
+        "renorm_checks": renorm_checks,
+        "renormalizations": renorms,
     }
     if counts is not None:
         metrics["counts"] = counts

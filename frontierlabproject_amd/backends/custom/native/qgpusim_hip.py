@@ -28,7 +28,7 @@ LIB_PATH = os.environ.get("QGPUSIM_LIB") or os.path.join(_PKG, "lib", "libqgpusi
 C64, C128 = 0, 1
 SEQUENTIAL, GRAPHED, COOPERATIVE = 0, 1, 2
 NOALLOC_SCRATCH_BYTES = 32 << 20  # QSIM_NOALLOC_SCRATCH_BYTES
-OP_1Q, OP_2Q, OP_PROBE = 1, 2, 3
+OP_1Q, OP_2Q, OP_PROBE, OP_3Q, OP_4Q = 1, 2, 3, 4, 5
 
 
 class QsimOp(ctypes.Structure):
@@ -192,6 +192,24 @@ def pack_ops(ops, num_qubits, dtype):
                 raise RuntimeError("apply_tile_dev: q0 == q1")
             m = _matrix(t[3], 2, dtype)
             arr[i].kind, arr[i].q0, arr[i].q1 = OP_2Q, q0, q1
+        elif kind in ("3q", "4q"):
+            # no reference counterpart (the reference drops these ops):
+            # ('3q', q0, q1, q2, U) / ('4q', q0, q1, q2, q3, U), matrix index
+            # sum_i b_{q_i} 2^i
+            k = 3 if kind == "3q" else 4
+            if len(t) != k + 2:
+                raise RuntimeError("apply_tile_dev: %s expects ('%s', %s, U_dev)"
+                                   % (kind, kind, ", ".join("q%d" % j for j in range(k))))
+            qs = [int(x) for x in t[1:k + 1]]
+            if any(q < 0 or q >= num_qubits for q in qs):
+                raise RuntimeError("apply_tile_dev: invalid qubit")
+            if len(set(qs)) != k:
+                raise RuntimeError("apply_tile_dev: repeated qubit")
+            m = _matrix(t[k + 1], k, dtype)
+            packed = 0
+            for j, q in enumerate(qs[1:]):
+                packed |= q << (8 * j)
+            arr[i].kind, arr[i].q0, arr[i].q1 = (OP_3Q if k == 3 else OP_4Q), qs[0], packed
         else:
             raise RuntimeError("apply_tile_dev: unknown op kind: " + kind)
         arr[i].mat_offset = off

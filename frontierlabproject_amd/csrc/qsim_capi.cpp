@@ -131,6 +131,7 @@ struct qsim_state {
     size_t small_cap = 0;          // doubles
     double *d_probe = nullptr;     // probe partials
     size_t probe_cap = 0;          // doubles
+    void *d_gatek = nullptr;       // matrix of a 3- or 4-qubit op in the sequential mode (16x16 complex128)
     // kernel launches, graph launches, graph captures, chunk passes,
     // permutations run, chunk passes run by specialised kernels, permutations
     // folded into a chunk pass
@@ -259,6 +260,7 @@ int ensure_probe(qsim_state *st, size_t doubles)
     if (doubles <= st->probe_cap) return 0;
     HIP_TRY(hipStreamSynchronize(st->stream));
     if (st->d_probe) hipFree(st->d_probe);
+    if (st->d_gatek) hipFree(st->d_gatek);
     st->d_probe = nullptr;
     HIP_TRY(hipMalloc(reinterpret_cast<void **>(&st->d_probe), doubles * sizeof(double)));
     st->probe_cap = doubles;
@@ -276,6 +278,22 @@ void drop_graphs(qsim_state *st)
     st->graphs.clear();
 }
 
+// Qubits of a QSIM_OP_3Q / QSIM_OP_4Q op: q0, then q1's 8-bit fields.
+int kq_qubits(const qsim_op &o, int *q)
+{
+    const int k = o.kind == QSIM_OP_3Q ? 3 : 4;
+    q[0] = o.q0;
+    for (int j = 1; j < k; ++j) q[j] = (int)(((uint32_t)o.q1 >> (8 * (j - 1))) & 255u);
+    return k;
+}
+
+int32_t kq_pack(const int *q, int k)
+{
+    uint32_t v = 0;
+    for (int j = 1; j < k; ++j) v |= (uint32_t)q[j] << (8 * (j - 1));
+    return (int32_t)v;
+}
+
 int check_ops(int n, const qsim_op *ops, int nops, uint64_t mat_count, bool allow_probe)
 {
     if (nops < 0) return fail("apply_tile_dev: negative op count");
@@ -290,6 +308,19 @@ int check_ops(int n, const qsim_op *ops, int nops, uint64_t mat_count, bool allo
             if (o.q0 < 0 || o.q0 >= n || o.q1 < 0 || o.q1 >= n) return fail("apply_tile_dev: invalid q0/q1");
             if (o.q0 == o.q1) return fail("apply_tile_dev: q0 == q1");
             if (o.mat_offset < 0 || (uint64_t)o.mat_offset + 16 > mat_count)
+                return fail("apply_tile_dev: matrix offset out of range");
+        } else if (o.kind == QSIM_OP_3Q || o.kind == QSIM_OP_4Q) {
+            int q[4];
+            const int k = kq_qubits(o, q);
+            if (o.q1 < 0 || (k == 3 && ((uint32_t)o.q1 >> 16) != 0u) || ((uint32_t)o.q1 >> 24) != 0u)
+                return fail("apply_tile_dev: invalid qubit field");
+            uint64_t seen = 0;
+            for (int j = 0; j < k; ++j) {
+                if (q[j] < 0 || q[j] >= n) return fail("apply_tile_dev: invalid qubit");
+                if ((seen >> q[j]) & 1) return fail("apply_tile_dev: repeated qubit");
+                seen |= 1ull << q[j];
+            }
+            if (o.mat_offset < 0 || (uint64_t)o.mat_offset + (1u << (2 * k)) > mat_count)
                 return fail("apply_tile_dev: matrix offset out of range");
         } else {
             return fail("apply_tile_dev: unknown op kind: " + std::to_string(o.kind));
@@ -307,6 +338,13 @@ std::vector<PlanOp> to_plan_ops(const qsim_op *ops, int nops)
         v[i].q1 = ops[i].kind == QSIM_OP_2Q ? ops[i].q1 : -1;
         v[i].mat_off = ops[i].mat_offset;
         v[i].probe = ops[i].kind == QSIM_OP_PROBE ? ops[i].q1 : -1;
+        if (ops[i].kind == QSIM_OP_3Q || ops[i].kind == QSIM_OP_4Q) {
+            int q[4];
+            const int k = kq_qubits(ops[i], q);
+            v[i].q1 = q[1];
+            v[i].q2 = q[2];
+            v[i].q3 = k == 4 ? q[3] : -1;
+        }
     }
     return v;
 }
@@ -323,6 +361,15 @@ int run_sequential(qsim_state *st, const qsim_op *ops, int nops, const void *mat
             Mat1<R> U;
             std::memcpy(U.u, m + o.mat_offset, sizeof(U));
             HIP_TRY(launch_1q<R>(psi, st->n, o.q0, U, st->stream));
+        } else if (o.kind == QSIM_OP_3Q || o.kind == QSIM_OP_4Q) {
+            // the matrix goes to a device buffer (stream-ordered copy: the
+            // previous op's kernel has read it before it is overwritten)
+            int q[4];
+            const int k = kq_qubits(o, q);
+            if (!st->d_gatek) HIP_TRY(hipMalloc(&st->d_gatek, 256 * sizeof(cx<double>)));
+            HIP_TRY(hipMemcpyAsync(st->d_gatek, m + o.mat_offset, sizeof(cx<R>) << (2 * k), hipMemcpyHostToDevice,
+                                   st->stream));
+            HIP_TRY(launch_gate_k<R>(psi, st->n, q, k, static_cast<const cx<R> *>(st->d_gatek), st->stream));
         } else {
             Mat2<R> U;
             std::memcpy(U.u, m + o.mat_offset, sizeof(U));
@@ -450,6 +497,11 @@ void choose_layout_impl(int n, int cmin, int region, const int32_t *at, const qs
     for (int i = 0; i < nops; ++i) {
         T |= 1ull << ops[i].q0;
         if (ops[i].kind == QSIM_OP_2Q) T |= 1ull << ops[i].q1;
+        if (ops[i].kind == QSIM_OP_3Q || ops[i].kind == QSIM_OP_4Q) {
+            int q[4];
+            const int k = kq_qubits(ops[i], q);
+            for (int j = 1; j < k; ++j) T |= 1ull << q[j];
+        }
     }
     const int r = std::min(n, std::max(__builtin_popcountll(T), region));
     std::vector<int> slots, in;  // ascending stored positions
@@ -542,6 +594,14 @@ int plan_and_compile(qsim_state *st, const qsim_op *ops, int nops, const void *m
     choose_layout(st, ops, nops, tgt);
     std::vector<qsim_op> mapped(ops, ops + nops);
     for (qsim_op &o : mapped) {
+        if (o.kind == QSIM_OP_3Q || o.kind == QSIM_OP_4Q) {
+            int q[4];
+            const int k = kq_qubits(o, q);
+            for (int j = 0; j < k; ++j) q[j] = tgt[q[j]];
+            o.q0 = q[0];
+            o.q1 = kq_pack(q, k);
+            continue;
+        }
         o.q0 = tgt[o.q0];
         if (o.kind == QSIM_OP_2Q) o.q1 = tgt[o.q1];
     }
@@ -1085,6 +1145,7 @@ int qsim_destroy(qsim_state *st)
     if (st->tmp) hipFree(st->tmp);
     if (st->d_small) hipFree(st->d_small);
     if (st->d_probe) hipFree(st->d_probe);
+    if (st->d_gatek) hipFree(st->d_gatek);
     for (auto &r : st->recs) {
         hipEventDestroy(r.a);
         hipEventDestroy(r.b);
@@ -1324,7 +1385,9 @@ int qsim_apply_2q_raw(void *psi, int dtype, int n, int q0, int q1, const void *U
 int qsim_norm2(qsim_state *st, double *out)
 {
     if (!st || !out) return fail("null argument");
-    if (flush_perm(st)) return -1;
+    // the norm does not depend on the stored bit order: a pending layout
+    // change stays pending (no permutation pass)
+    if (materialize_zero(st)) return -1;
     if (st->dtype == QSIM_C64) return dot_t<float>(st, st->psi, st->psi, out, nullptr);
     return dot_t<double>(st, st->psi, st->psi, out, nullptr);
 }
@@ -1447,6 +1510,10 @@ int qsim_choose_layout(int num_qubits, int dtype, int cmin, const int32_t *perm_
     for (int i = 0; i < nops; ++i) {
         const qsim_op &o = ops[i];
         const bool two = o.kind == QSIM_OP_2Q;
+        if (o.kind == QSIM_OP_3Q || o.kind == QSIM_OP_4Q) {
+            if (check_ops(num_qubits, &o, 1, UINT64_MAX, false)) return fail("invalid op");
+            continue;
+        }
         if ((o.kind != QSIM_OP_1Q && !two && o.kind != QSIM_OP_PROBE) || o.q0 < 0 || o.q0 >= num_qubits ||
             (two && (o.q1 < 0 || o.q1 >= num_qubits || o.q1 == o.q0)))
             return fail("invalid op");

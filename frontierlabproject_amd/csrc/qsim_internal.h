@@ -71,6 +71,10 @@ hipError_t launch_sum_partials(const double *partial, int nblocks, int width, do
                                hipStream_t s);
 template <typename R>
 hipError_t launch_scale(cx<R> *psi, uint64_t dim, double factor, hipStream_t s);
+// Dense gate on k = 3 or 4 qubits q[0..k-1] (matrix index sum_i b_{q_i} 2^i),
+// matrix in device memory; n >= k.
+template <typename R>
+hipError_t launch_gate_k(cx<R> *psi, int n, const int *q, int k, const cx<R> *d_U, hipStream_t s);
 
 int max_dot_blocks();
 hipError_t prepare_kernels();

@@ -4,6 +4,8 @@
 // Index convention: amplitude index i = sum_q b_q 2^q; a 2-qubit matrix acts
 // little-endian over (q0, q1): row/col = b_q0 + 2 b_q1
 // (qgpusim_cuda/src/kernels.cu:54-55).
+#include <algorithm>
+
 #include "qsim_internal.h"
 
 namespace qsim {
@@ -402,6 +404,60 @@ hipError_t launch_permute(const cx<R> *in, cx<R> *out, const PermArgs &P, hipStr
 
 // One chunk per block.  complex64 forward passes ask for 4 waves per SIMD
 // (at most 128 VGPRs): the copy batches otherwise push occupancy to 3.
+// k-qubit dense gate (k = 3, 4) as one coalesced stream: thread t updates
+// the 2^k amplitudes of group t (zeros inserted at the op's qubits in
+// ascending order; member m at sum_i b_i(m) 2^{q_i}) with the row-major
+// matrix U (index sum_i b_{q_i} 2^i), read through the scalar cache.
+struct GateK {
+    int32_t q[4];        // the op's qubits, matrix-index order
+    int32_t sorted[4];   // the same, ascending
+};
+
+template <typename R, int K>
+__global__ __launch_bounds__(kTPB) void k_gate_k(cx<R> *__restrict__ psi, uint64_t groups, GateK g,
+                                                 const cx<R> *__restrict__ U)
+{
+    const uint64_t t = (uint64_t)blockIdx.x * kTPB + threadIdx.x;
+    if (t >= groups) return;
+    uint64_t base = t;
+#pragma unroll
+    for (int j = 0; j < K; ++j) base = insert_zero(base, g.sorted[j]);
+    constexpr int M = 1 << K;
+    cx<R> a[M];
+#pragma unroll
+    for (int m = 0; m < M; ++m) {
+        uint64_t off = 0;
+#pragma unroll
+        for (int i = 0; i < K; ++i) off |= (uint64_t)((m >> i) & 1) << g.q[i];
+        a[m] = psi[base + off];
+    }
+#pragma unroll
+    for (int r = 0; r < M; ++r) {
+        cx<R> acc = cmul(U[M * r], a[0]);
+#pragma unroll
+        for (int c = 1; c < M; ++c) acc = cmad(U[M * r + c], a[c], acc);
+        uint64_t off = 0;
+#pragma unroll
+        for (int i = 0; i < K; ++i) off |= (uint64_t)((r >> i) & 1) << g.q[i];
+        psi[base + off] = acc;
+    }
+}
+
+template <typename R>
+hipError_t launch_gate_k(cx<R> *psi, int n, const int *q, int k, const cx<R> *d_U, hipStream_t s)
+{
+    GateK g;
+    for (int j = 0; j < 4; ++j) g.q[j] = g.sorted[j] = j < k ? q[j] : 0;
+    std::sort(g.sorted, g.sorted + k);
+    const uint64_t groups = 1ull << (n - k);
+    const unsigned blocks = (unsigned)((groups + kTPB - 1) / kTPB);
+    if (k == 3)
+        hipLaunchKernelGGL((k_gate_k<R, 3>), dim3(blocks), dim3(kTPB), 0, s, psi, groups, g, d_U);
+    else
+        hipLaunchKernelGGL((k_gate_k<R, 4>), dim3(blocks), dim3(kTPB), 0, s, psi, groups, g, d_U);
+    return hipGetLastError();
+}
+
 template <typename R, int NV, bool GENERIC>
 __global__ __launch_bounds__(kTPB)
 __attribute__((amdgpu_waves_per_eu((NV == 1 && sizeof(R) == 4) ? 4 : 1))) void k_tile_pass(
@@ -705,7 +761,8 @@ hipError_t prepare_kernels()
     template hipError_t launch_pauli_chunk<R>(const cx<R> *, cx<R> *, int, const PassDev &,                 \
                                               const PassDev *, const PauliDev *, int, int, double *,        \
                                               hipStream_t);                                                 \
-    template hipError_t launch_scale<R>(cx<R> *, uint64_t, double, hipStream_t);
+    template hipError_t launch_scale<R>(cx<R> *, uint64_t, double, hipStream_t);                           \
+    template hipError_t launch_gate_k<R>(cx<R> *, int, const int *, int, const cx<R> *, hipStream_t);
 INST(float)
 INST(double)
 #undef INST

@@ -117,8 +117,7 @@ uint64_t search_chunk(const PlanOp *ops, const std::vector<int> &rem, uint64_t P
     masks.reserve(rem.size());
     uint64_t Q = 0;
     for (int idx : rem) {
-        uint64_t m = 1ull << ops[idx].q0;
-        if (ops[idx].kind == OP_2Q) m |= 1ull << ops[idx].q1;
+        const uint64_t m = op_mask(ops[idx]);
         masks.push_back(m);
         Q |= m;
     }
@@ -375,6 +374,7 @@ int place(Ctx &c, StepBuilder &sb, int idx, int rx, int ry)
 {
     const PlanOp &op = c.ops[idx];
     Plan &p = *c.plan;
+    if (op.kind == OP_3Q || op.kind == OP_4Q) return 2;  // a dense step
     if (op.kind == OP_1Q || op.kind == OP_PROBE) {
         const int t = rx;
         Cd u[4];
@@ -472,20 +472,22 @@ int place(Ctx &c, StepBuilder &sb, int idx, int rx, int ry)
     return 1;
 }
 
-// 16x16 embedding of op `idx` on register bits (rx, ry): out = E * in.
-void embed(Ctx &c, int idx, int rx, int ry, Cd *E)
+// 16x16 embedding of op `idx` on register bits reg[0..k-1] (its qubits in
+// matrix-index order): out = E * in.
+void embed(Ctx &c, int idx, const int *reg, Cd *E)
 {
     const PlanOp &op = c.ops[idx];
-    for (int k = 0; k < 256; ++k) E[k] = {0.0, 0.0};
-    const bool two = op.kind == OP_2Q;
-    const int k = two ? 2 : 1;
-    const int mask = (1 << rx) | (two ? (1 << ry) : 0);
+    int q[4];
+    const int k = op_qubits(op, q);
+    for (int i = 0; i < 256; ++i) E[i] = {0.0, 0.0};
+    int mask = 0;
+    for (int j = 0; j < k; ++j) mask |= 1 << reg[j];
     for (int col = 0; col < 16; ++col) {
-        const int ci = ((col >> rx) & 1) | (two ? ((col >> ry) & 1) << 1 : 0);
+        int ci = 0;
+        for (int j = 0; j < k; ++j) ci |= ((col >> reg[j]) & 1) << j;
         for (int ri = 0; ri < (1 << k); ++ri) {
             int row = col & ~mask;
-            row |= (ri & 1) << rx;
-            if (two) row |= ((ri >> 1) & 1) << ry;
+            for (int j = 0; j < k; ++j) row |= ((ri >> j) & 1) << reg[j];
             E[16 * row + col] = mat_at(c.mats, c.eb, (int64_t)op.mat_off + (1 << k) * ri + ci);
         }
     }
@@ -505,11 +507,12 @@ void matmul16(const Cd *A, const Cd *B, Cd *out)  // out = A * B
         }
 }
 
-// Fold op `idx` into the dense matrix of the step.
-void fold_dense(Ctx &c, StepBuilder &sb, int idx, int rx, int ry)
+// Fold op `idx` (its qubits on register bits reg[]) into the dense matrix of
+// the step.
+void fold_dense(Ctx &c, StepBuilder &sb, int idx, const int *reg)
 {
     Cd E[256], T[256];
-    embed(c, idx, rx, ry, E);
+    embed(c, idx, reg, E);
     matmul16(E, sb.M, T);
     std::memcpy(sb.M, T, sizeof(T));
     sb.placed.push_back({(int)sb.placed.size(), idx});
@@ -526,12 +529,10 @@ void to_dense(Ctx &c, StepBuilder &sb)
     sb.factor = {1.0, 0.0};
     for (auto &pr : seq) {
         const PlanOp &op = c.ops[pr.second];
-        int rx = -1, ry = -1;
-        for (int t = 0; t < kG; ++t) {
-            if (sb.regbit[t] == c.loc[op.q0]) rx = t;
-            if (op.kind == OP_2Q && sb.regbit[t] == c.loc[op.q1]) ry = t;
-        }
-        fold_dense(c, sb, pr.second, rx, ry);
+        int q[4], reg[4];
+        const int k = op_qubits(op, q);
+        for (int j = 0; j < k; ++j) reg[j] = sb.reg_of(c.loc[q[j]]);
+        fold_dense(c, sb, pr.second, reg);
     }
     for (int j = 0; j < kSlots; ++j) sb.sd.kind[j] = 0;
 }
@@ -564,16 +565,26 @@ bool plan_passes(int n, const PlanOp *ops, int nops, const void *mats, int elem_
     if (max_probes < 1) max_probes = 1;
     for (int i = 0; i < nops; ++i) {
         const PlanOp &op = ops[i];
-        if (op.kind != OP_1Q && op.kind != OP_2Q && op.kind != OP_PROBE) {
+        if (op.kind != OP_1Q && op.kind != OP_2Q && op.kind != OP_PROBE && op.kind != OP_3Q && op.kind != OP_4Q) {
             err = "tile plan: unknown op kind";
             return false;
         }
-        if (op.q0 < 0 || op.q0 >= n || (op.kind == OP_2Q && (op.q1 < 0 || op.q1 >= n))) {
-            err = "tile plan: qubit out of range";
-            return false;
+        int q[4];
+        const int k = op_qubits(op, q);
+        uint64_t seen = 0;
+        for (int j = 0; j < k; ++j) {
+            if (q[j] < 0 || q[j] >= n) {
+                err = "tile plan: qubit out of range";
+                return false;
+            }
+            if ((seen >> q[j]) & 1) {
+                err = k == 2 ? "tile plan: q0 == q1" : "tile plan: repeated qubit";
+                return false;
+            }
+            seen |= 1ull << q[j];
         }
-        if (op.kind == OP_2Q && op.q0 == op.q1) {
-            err = "tile plan: q0 == q1";
+        if (k > 2 && K < k) {
+            err = "tile plan: chunk smaller than a multi-qubit op";
             return false;
         }
     }
@@ -631,8 +642,7 @@ bool plan_passes(int n, const PlanOp *ops, int nops, const void *mats, int elem_
         std::vector<int> chosen, left;
         for (int idx : rem) {
             const PlanOp &op = ops[idx];
-            uint64_t m = 1ull << op.q0;
-            if (op.kind == OP_2Q) m |= 1ull << op.q1;
+            const uint64_t m = op_mask(op);
             const bool probe = op.kind == OP_PROBE;
             if ((m & blocked) || popc(P | m) > Kp || (fixed && (m & ~fixed)) || (probe && nprobe >= max_probes)) {
                 blocked |= m;
@@ -667,10 +677,13 @@ bool plan_passes(int n, const PlanOp *ops, int nops, const void *mats, int elem_
             std::vector<int> sleft;
             for (int idx : srem) {
                 const PlanOp &op = ops[idx];
-                const int c0 = loc[op.q0];
-                const int c1 = op.kind == OP_2Q ? loc[op.q1] : -1;
-                uint64_t m = 1ull << c0;
-                if (c1 >= 0) m |= 1ull << c1;
+                int q[4], cb[4], reg[4];
+                const int k = op_qubits(op, q);
+                uint64_t m = 0;
+                for (int j = 0; j < k; ++j) {
+                    cb[j] = loc[q[j]];
+                    m |= 1ull << cb[j];
+                }
                 if (m & sblk) {
                     sblk |= m;
                     sleft.push_back(idx);
@@ -679,33 +692,27 @@ bool plan_passes(int n, const PlanOp *ops, int nops, const void *mats, int elem_
                 StepBuilder save = sb;
                 const size_t mats_before = out.mats.size(), ids_before = out.probe_ids.size();
                 const int np_before = pd.nprobe;
-                int rx = sb.reg_of(c0), ry = c1 >= 0 ? sb.reg_of(c1) : -1;
                 bool ok = true;
-                if (rx < 0) {
-                    if (sb.nreg < kG) {
-                        rx = sb.nreg;
-                        sb.regbit[sb.nreg++] = c0;
-                    } else {
-                        ok = false;
-                    }
-                }
-                if (ok && c1 >= 0 && ry < 0) {
-                    if (sb.nreg < kG) {
-                        ry = sb.nreg;
-                        sb.regbit[sb.nreg++] = c1;
-                    } else {
-                        ok = false;
+                for (int j = 0; j < k && ok; ++j) {
+                    reg[j] = sb.reg_of(cb[j]);
+                    if (reg[j] < 0) {
+                        if (sb.nreg < kG) {
+                            reg[j] = sb.nreg;
+                            sb.regbit[sb.nreg++] = cb[j];
+                        } else {
+                            ok = false;
+                        }
                     }
                 }
                 if (ok) {
                     if (sb.dense) {
                         ok = op.kind != OP_PROBE;
-                        if (ok) fold_dense(ctx, sb, idx, rx, ry);
+                        if (ok) fold_dense(ctx, sb, idx, reg);
                     } else {
-                        const int r = place(ctx, sb, idx, rx, ry);
+                        const int r = place(ctx, sb, idx, reg[0], k > 1 ? reg[1] : -1);
                         if (r == 2 && !sb.has_probe) {
                             to_dense(ctx, sb);
-                            fold_dense(ctx, sb, idx, rx, ry);
+                            fold_dense(ctx, sb, idx, reg);
                         } else {
                             ok = r == 1;
                         }

@@ -32,6 +32,8 @@ enum OpKind : int32_t {
     OP_1Q = 1,      // 2x2 matrix on bit q0
     OP_2Q = 2,      // 4x4 matrix on (q0, q1), index b_q0 + 2 b_q1
     OP_PROBE = 3,   // accumulate Im<lam| G |psi> for the 2x2 matrix G on q0
+    OP_3Q = 4,      // 8x8 matrix on (q0, q1, q2), index b_q0 + 2 b_q1 + 4 b_q2 (a dense step)
+    OP_4Q = 5,      // 16x16 matrix on (q0, q1, q2, q3) (a dense step)
 };
 
 struct PlanOp {
@@ -39,7 +41,30 @@ struct PlanOp {
     int32_t q0, q1;      // bit positions in the state
     int32_t mat_off;     // complex-element offset of the matrix in the caller's arena
     int32_t probe;       // OP_PROBE: index of the output slot, else -1
+    int32_t q2 = -1, q3 = -1;  // third and fourth qubits of OP_3Q / OP_4Q
 };
+
+// Qubits an op acts on (1 to 4), in its matrix's index order.
+inline int op_qubits(const PlanOp &op, int *q)
+{
+    q[0] = op.q0;
+    if (op.kind == OP_1Q || op.kind == OP_PROBE) return 1;
+    q[1] = op.q1;
+    if (op.kind == OP_2Q) return 2;
+    q[2] = op.q2;
+    if (op.kind == OP_3Q) return 3;
+    q[3] = op.q3;
+    return 4;
+}
+
+inline uint64_t op_mask(const PlanOp &op)
+{
+    int q[4];
+    const int k = op_qubits(op, q);
+    uint64_t m = 0;
+    for (int i = 0; i < k; ++i) m |= 1ull << q[i];
+    return m;
+}
 
 struct Plan {
     std::vector<PassDev> passes;

@@ -29,14 +29,19 @@ enum { QSIM_C64 = 0, QSIM_C128 = 1 };
 /* tile_mode of run_custom_backend (qgpusim/backends/custom/backend.py:460-473) */
 enum { QSIM_SEQUENTIAL = 0, QSIM_GRAPHED = 1, QSIM_COOPERATIVE = 2 };
 
-enum { QSIM_OP_1Q = 1, QSIM_OP_2Q = 2, QSIM_OP_PROBE = 3 };
+enum { QSIM_OP_1Q = 1, QSIM_OP_2Q = 2, QSIM_OP_PROBE = 3, QSIM_OP_3Q = 4, QSIM_OP_4Q = 5 };
 
 typedef struct qsim_state qsim_state;
 
 /* One op of a tile.  Replaces the ('1q', q0, U_dev) / ('2q', q0, q1, U_dev)
  * tuples parsed at bindings.cpp:182-220: the matrix is `mat_offset` complex
  * elements into the matrix arena passed beside the ops.  QSIM_OP_PROBE (only
- * for qsim_adjoint) uses q1 as its output slot. */
+ * for qsim_adjoint) uses q1 as its output slot.  QSIM_OP_3Q / QSIM_OP_4Q (no
+ * reference counterpart: the reference drops gates on three or more qubits,
+ * qgpusim/backends/custom/backend.py:448-451) apply an 8x8 / 16x16 matrix to
+ * qubits (q0, q1 & 255, (q1 >> 8) & 255[, (q1 >> 16) & 255]), row and column
+ * index sum_i b_{q_i} 2^i; in the chunked modes each runs as one dense
+ * register step. */
 typedef struct {
     int32_t kind, q0, q1, mat_offset;
 } qsim_op;
@@ -151,7 +156,9 @@ int qsim_apply_2q_raw(void *psi, int dtype, int num_qubits, int q0, int q1, cons
 
 /* ---- No counterpart in the reference: required by the benchmark workloads. ---- */
 
-/* <psi|psi> in FP64 (Neumaier-compensated, fixed order).  Synchronises. */
+/* <psi|psi> in FP64 (Neumaier-compensated, fixed order).  Synchronises.  A
+ * pending layout change stays pending (the norm does not depend on it): the
+ * call reads the state once and writes nothing. */
 int qsim_norm2(qsim_state *st, double *out);
 
 /* <a|b> in FP64.  Both states need the same size; they may differ in

@@ -36,7 +36,8 @@ def test_little_endian_cx():
     assert np.argmax(abs(G.cx() @ v)) == 2
 
 
-@pytest.mark.parametrize("qargs", [[0], [3], [1, 4], [4, 1], [0, 5], [2, 3]])
+@pytest.mark.parametrize("qargs", [[0], [3], [1, 4], [4, 1], [0, 5], [2, 3], [5, 0, 3], [1, 2, 0],
+                                   [4, 0, 5, 2], [0, 1, 2, 3]])
 def test_dense_apply_matches_brute_force(qargs):
     rng = np.random.default_rng(7)
     n = 6
