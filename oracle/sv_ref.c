@@ -9,7 +9,8 @@
  *   2q quad  base = zero bits inserted at low and high positions, matrix index
  *            b_q0 + 2*b_q1 (kernels.cu:86-114, 132-136)
  *   permute  j = sum_p bit_p(i) << map[p]; out[j] = in[i] (kernels.cu:176-184)
- * Parallelised with OpenMP over pairs/quads/amplitudes.
+ * Parallelised with OpenMP over pairs/quads/amplitudes; the tiled run
+ * (svref_run_tiled_*) over blocks of the state.
  */
 #include <stdint.h>
 #include <string.h>
@@ -83,6 +84,73 @@ int svref_run_##SUF(T *psi, int n, const svref_op *ops, int nops, const T *mats)
         if (ops[i].kind == 1) svref_apply_1q_##SUF(psi, n, ops[i].q0, U);      \
         else if (ops[i].kind == 2) svref_apply_2q_##SUF(psi, n, ops[i].q0, ops[i].q1, U); \
         else return -1;                                                        \
+    }                                                                          \
+    return nops;                                                               \
+}                                                                              \
+                                                                               \
+/* One block of 2^b amplitudes (serial): 1q pair and 2q quad updates as      \
+ * above, with every qubit below b. */                                         \
+static void block_1q_##SUF(T *psi, int b, int q, const T *U)                   \
+{                                                                              \
+    const uint64_t stride = 1ULL << q, pairs = (1ULL << b) >> 1;               \
+    for (uint64_t k = 0; k < pairs; ++k) {                                     \
+        uint64_t i0 = (k / stride) * 2 * stride + k % stride, i1 = i0 + stride; \
+        T ar = psi[2 * i0], ai = psi[2 * i0 + 1];                              \
+        T br = psi[2 * i1], bi = psi[2 * i1 + 1];                              \
+        psi[2 * i0]     = U[0] * ar - U[1] * ai + U[2] * br - U[3] * bi;       \
+        psi[2 * i0 + 1] = U[0] * ai + U[1] * ar + U[2] * bi + U[3] * br;       \
+        psi[2 * i1]     = U[4] * ar - U[5] * ai + U[6] * br - U[7] * bi;       \
+        psi[2 * i1 + 1] = U[4] * ai + U[5] * ar + U[6] * bi + U[7] * br;       \
+    }                                                                          \
+}                                                                              \
+                                                                               \
+static void block_2q_##SUF(T *psi, int b, int q0, int q1, const T *U)          \
+{                                                                              \
+    const int lo = q0 < q1 ? q0 : q1, hi = q0 < q1 ? q1 : q0;                  \
+    const uint64_t mlo = 1ULL << lo, quads = (1ULL << b) >> 2;                 \
+    const uint64_t m0 = 1ULL << q0, m1 = 1ULL << q1;                           \
+    for (uint64_t k = 0; k < quads; ++k) {                                     \
+        uint64_t base = (k & (mlo - 1))                                        \
+                      | (((k >> lo) & ((1ULL << (hi - lo - 1)) - 1)) << (lo + 1)) \
+                      | ((k >> (hi - 1)) << (hi + 1));                         \
+        uint64_t idx[4] = {base, base | m0, base | m1, base | m0 | m1};        \
+        T vr[4], vi[4];                                                        \
+        for (int m = 0; m < 4; ++m) { vr[m] = psi[2 * idx[m]]; vi[m] = psi[2 * idx[m] + 1]; } \
+        for (int r = 0; r < 4; ++r) {                                          \
+            T sr = 0, si = 0;                                                  \
+            for (int c = 0; c < 4; ++c) {                                      \
+                T ur = U[2 * (4 * r + c)], ui = U[2 * (4 * r + c) + 1];        \
+                sr += ur * vr[c] - ui * vi[c];                                 \
+                si += ur * vi[c] + ui * vr[c];                                 \
+            }                                                                  \
+            psi[2 * idx[r]] = sr; psi[2 * idx[r] + 1] = si;                    \
+        }                                                                      \
+    }                                                                          \
+}                                                                              \
+                                                                               \
+/* Tiled run (the CPU counterpart of an LQR tile whose local qubits sit in   \
+ * the low bits): every op's qubits are below block_bits, so the state splits \
+ * into blocks of 2^block_bits contiguous amplitudes, and each block runs the  \
+ * whole op list while it is in the cache (one pass over memory for the list; \
+ * blocks in parallel).  Returns nops, or -1 when an op reaches a qubit at or \
+ * above block_bits or block_bits exceeds n. */                                \
+int svref_run_tiled_##SUF(T *psi, int n, const svref_op *ops, int nops, const T *mats, int block_bits) \
+{                                                                              \
+    if (block_bits > n || block_bits < 2) return -1;                           \
+    for (int i = 0; i < nops; ++i) {                                           \
+        if (ops[i].kind != 1 && ops[i].kind != 2) return -1;                   \
+        if (ops[i].q0 < 0 || ops[i].q0 >= block_bits) return -1;               \
+        if (ops[i].kind == 2 && (ops[i].q1 < 0 || ops[i].q1 >= block_bits)) return -1; \
+    }                                                                          \
+    const int64_t nblocks = (int64_t)1 << (n - block_bits);                    \
+    _Pragma("omp parallel for schedule(static)")                               \
+    for (int64_t blk = 0; blk < nblocks; ++blk) {                              \
+        T *p = psi + 2 * ((uint64_t)blk << block_bits);                        \
+        for (int i = 0; i < nops; ++i) {                                       \
+            const T *U = mats + 2 * (int64_t)ops[i].mat_offset;                \
+            if (ops[i].kind == 1) block_1q_##SUF(p, block_bits, ops[i].q0, U);  \
+            else block_2q_##SUF(p, block_bits, ops[i].q0, ops[i].q1, U);       \
+        }                                                                      \
     }                                                                          \
     return nops;                                                               \
 }

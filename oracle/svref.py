@@ -35,6 +35,9 @@ def lib():
             getattr(_lib, f"svref_run_{suf}").argtypes = [
                 ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
             getattr(_lib, f"svref_run_{suf}").restype = ctypes.c_int
+            getattr(_lib, f"svref_run_tiled_{suf}").argtypes = [
+                ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int]
+            getattr(_lib, f"svref_run_tiled_{suf}").restype = ctypes.c_int
     return _lib
 
 
@@ -71,6 +74,18 @@ def run(psi, n, ops):
                                             len(ops), arena.ctypes.data)
     if r < 0:
         raise ValueError("svref: op with more than two qubits")
+    return psi
+
+
+def run_tiled(psi, n, ops, block_bits):
+    """Apply ``[(matrix, qargs)]`` in place, every qarg below ``block_bits``:
+    blocks of 2^block_bits contiguous amplitudes each run the whole list."""
+    suf = _suffix(psi)
+    arr, arena = pack(ops, psi.dtype)
+    r = getattr(lib(), f"svref_run_tiled_{suf}")(psi.ctypes.data, n, ctypes.addressof(arr), len(ops),
+                                                  arena.ctypes.data, block_bits)
+    if r < 0:
+        raise ValueError("svref: an op reaches a qubit outside the block, or has more than two qubits")
     return psi
 
 

@@ -189,3 +189,19 @@ def test_random_generator_counts():
     two = [o for o in ops if len(o.qubits) == 2]
     assert len(one) == 400
     assert 60 < len(two) < 140
+
+
+@pytest.mark.parametrize("dtype,tol", [(np.complex128, 1e-13), (np.complex64, 2e-6)])
+def test_tiled_c_port_matches_dense(dtype, tol):
+    """The C port's tiled run (blocks of 2^9 amplitudes each running the whole
+    list) equals program-order FP64 simulation; an op outside the block is
+    refused."""
+    n, b = 13, 9
+    ops = [(U, q) for U, q in C.to_dense_gates(C.random_circuit(n, 8, 31)) if max(q) < b]
+    psi0 = C.random_state(n, 2)
+    ref = dense.simulate(ops, n, psi0)
+    got = psi0.astype(dtype)
+    svref.run_tiled(got, n, ops, b)
+    assert np.max(np.abs(got - ref)) < tol
+    with pytest.raises(ValueError):
+        svref.run_tiled(psi0.astype(dtype), n, [(G.x(), [b])], b)
