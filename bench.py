@@ -500,7 +500,8 @@ def other_legs(args):
 def fused_legs(hip, sv, prog, args, bytes_per_pass, ref_state):
     """Both fusion modes; ref_state (the plain path's state) or None."""
     out = {"fused": fused_leg(hip, sv, prog, args, bytes_per_pass, ref_state, 1),
-           "fused_selective": fused_leg(hip, sv, prog, args, bytes_per_pass, ref_state, 2)}
+           "fused_selective": fused_leg(hip, sv, prog, args, bytes_per_pass, ref_state, 2),
+           "fused5": fused_leg(hip, sv, prog, args, bytes_per_pass, ref_state, 3)}
     sv.set_fusion(0)
     return out
 
@@ -508,7 +509,8 @@ def fused_legs(hip, sv, prog, args, bytes_per_pass, ref_state):
 def fused_leg(hip, sv, prog, args, bytes_per_pass, ref_state, mode):
     """The same program with register steps fused into 4-qubit unitaries
     applied as complex GEMMs on the FP32 matrix cores (3M): every step (mode
-    1), or those whose gates cost more than the product (mode 2)."""
+    1), or those whose gates cost more than the product (mode 2); or every
+    step fused into a 5-qubit (32x32) unitary (mode 3)."""
     sv.set_fusion(mode)
     for _ in range(max(1, args.warmup)):
         run_program(hip, sv, prog, hip.GRAPHED)

@@ -424,7 +424,9 @@ class _StatevectorBase:
         """0: slot programs (a step fused only when its gates cost more than a
         dense product on the vector units); 1: every register step one fused
         4-qubit unitary on the matrix cores (complex64, specialised kernels);
-        2: a step fused when its gates cost more than that matrix-core product."""
+        2: a step fused when its gates cost more than that matrix-core product;
+        3: every register step one fused 5-qubit (32x32) unitary on the matrix
+        cores (complex64 with specialised kernels; otherwise as mode 1)."""
         _check(lib.qsim_set_fusion(self._h, int(mode)))
 
     def set_jit(self, enable=True):

@@ -419,6 +419,9 @@ int enqueue_passes(qsim_state *st, const Plan &plan, const DescImage &img, char 
                                           1, 1, 0, st->stream, lam ? args2 : args1, nullptr));
             st->counters[5]++;
         } else {
+            // 32x32 dense steps exist only in specialised kernels
+            for (int si = hp.step_off; si < hp.step_off + hp.nsteps; ++si)
+                if (plan.steps[si].bit4 >= 0) return fail("internal: 32x32 dense step without its specialised kernel");
             HIP_TRY(launch_tile_pass<R>(static_cast<cx<R> *>(st->psi), static_cast<cx<R> *>(lam), st->n, hp, dp + p,
                                         ds, dm, st->d_probe, plan.generic, st->stream));
         }
@@ -1465,7 +1468,7 @@ int qsim_set_jit(qsim_state *st, int enable)
 int qsim_set_fusion(qsim_state *st, int mode)
 {
     if (!st) return fail("null state");
-    if (mode < 0 || mode > 2) return fail("fusion mode must be 0, 1 or 2");
+    if (mode < 0 || mode > 3) return fail("fusion mode must be 0, 1, 2 or 3");
     st->fuse = mode;
     return 0;
 }

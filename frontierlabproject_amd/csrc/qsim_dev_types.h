@@ -120,7 +120,7 @@ struct StepDev {
     int32_t nrounds;             // slot rounds in use
     int32_t dense;               // arena offset of a 16x16 matrix (dense step) or -1
     int32_t scale;               // arena offset of a factor applied after the step, or -1
-    int32_t pad;
+    int32_t bit4;                // fifth register bit of a 32x32 dense step (register index 4), or -1
     int32_t kind[kSlots];        // slot flags
     int32_t mat[kSlots];         // complex-element offsets into the plan's arena
     int32_t probe[k1qSlots];     // probe slot within the pass, per 1q slot

@@ -863,6 +863,65 @@ __device__ __forceinline__ void dense16_mfma(cx<float> *c0, MP M, const int *mof
     }
 }
 
+// Dense 32x32 step (five register bits) of a complex64 chunk on the matrix
+// cores: X (32 x groups) <- M X as two 16-row halves, each the sum of eight
+// 16x16x4 k-blocks of the three 3M products.  Lane j + 16 q reads rows
+// 4 kb + q (kb = 0..7) of group (block base + j) and writes rows
+// 16 h + 4 q + r of it, all reads before any write.  Waves take blocks of 16
+// groups in turn, so a chunk of 2^12 amplitudes (128 groups) keeps every wave
+// of a 256-thread workgroup busy.  moff[m]: LDS slot offset of register index
+// m (0..31); s0..s4: the register bits ascending; ng: groups in the chunk.
+template <int TPB = kTPB, typename MP = const cx<float> *>
+__device__ __forceinline__ void dense32_mfma(cx<float> *c0, MP M, const int *moff, int s0, int s1, int s2, int s3,
+                                             int s4, int ng, MP sc)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int j = lane & 15, q = lane >> 4;
+    float ar[2][8], ai[2][8], as[2][8];
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+#pragma unroll
+        for (int kb = 0; kb < 8; ++kb) {
+            const cx<float> m = ldc(M, 32 * (16 * h + j) + 4 * kb + q);
+            ar[h][kb] = m.re;
+            ai[h][kb] = m.im;
+            as[h][kb] = m.re + m.im;
+        }
+    const cx<float> f = sc ? ldc(sc, 0) : cx<float>{1.f, 0.f};
+    for (int gb = wave * 16; gb < ng; gb += TPB / 4) {  // wave-uniform trip count
+        const int g = gb + j;
+        const bool act = g < ng;
+        const int sb =
+            swz((int)ins0(ins0(ins0(ins0(ins0((uint32_t)(act ? g : ng - 1), s0), s1), s2), s3), s4));
+        float xr[8], xi[8], xs[8];
+#pragma unroll
+        for (int kb = 0; kb < 8; ++kb) {
+            const cx<float> v = c0[sb ^ moff[4 * kb + q]];
+            xr[kb] = v.re;
+            xi[kb] = v.im;
+            xs[kb] = v.re + v.im;
+        }
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            v4f p1 = {0.f, 0.f, 0.f, 0.f}, p2 = p1, p3 = p1;
+#pragma unroll
+            for (int kb = 0; kb < 8; ++kb) {
+                p1 = __builtin_amdgcn_mfma_f32_16x16x4f32(ar[h][kb], xr[kb], p1, 0, 0, 0);
+                p2 = __builtin_amdgcn_mfma_f32_16x16x4f32(ai[h][kb], xi[kb], p2, 0, 0, 0);
+                p3 = __builtin_amdgcn_mfma_f32_16x16x4f32(as[h][kb], xs[kb], p3, 0, 0, 0);
+            }
+            if (act) {
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    cx<float> o{p1[r] - p2[r], p3[r] - p1[r] - p2[r]};
+                    if (sc) o = cmul(o, f);
+                    c0[sb ^ moff[16 * h + 4 * q + r]] = o;
+                }
+            }
+        }
+    }
+}
+
 // FP64 reductions with Neumaier-compensated per-thread sums; fixed order,
 // so results are deterministic.
 struct NSum {

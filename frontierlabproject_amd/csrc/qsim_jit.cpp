@@ -52,6 +52,25 @@ void emit_step(std::ostringstream &o, const StepDev &sd, int E, const char *T, i
         moff[m] = swz_host(((m & 1) << sd.bits[0]) | (((m >> 1) & 1) << sd.bits[1]) |
                            (((m >> 2) & 1) << sd.bits[2]) | (((m >> 3) & 1) << sd.bits[3]));
     const bool loop = NG > tpb, guard = NG < tpb;
+    if (sd.bit4 >= 0) {  // 32x32 on the matrix cores (complex64 one-vector passes only)
+        int bits5[5] = {sd.bits[0], sd.bits[1], sd.bits[2], sd.bits[3], sd.bit4};
+        int sorted5[5];
+        std::copy(bits5, bits5 + 5, sorted5);
+        std::sort(sorted5, sorted5 + 5);
+        o << "  { const int moff[32] = {";
+        for (int m = 0; m < 32; ++m) {
+            int off = 0;
+            for (int t = 0; t < 5; ++t) off |= ((m >> t) & 1) << bits5[t];
+            o << swz_host(off) << (m < 31 ? ", " : "");
+        }
+        o << "};\n    dense32_mfma<" << tpb << ">(c0, mats + " << sd.dense << ", moff";
+        for (int t = 0; t < 5; ++t) o << ", " << sorted5[t];
+        o << ", " << (E >> 5) << ", "
+          << (sd.scale >= 0 ? "mats + " + std::to_string(sd.scale) : std::string("(const cx<float> *)nullptr"))
+          << ");\n  }\n";
+        o << "  __syncthreads();\n";
+        return;
+    }
     if (sd.dense >= 0 && nv == 1 && std::string(T) == "float") {  // matrix cores
         o << "  { const int moff[16] = {";
         for (int m = 0; m < 16; ++m) o << moff[m] << (m < 15 ? ", " : "");

@@ -205,25 +205,30 @@ int next_slot(int last, int target)
     return j < kSlots ? j : -1;
 }
 
+constexpr int kG5 = kG + 1;  // register bits of a 32x32 dense step
+
 struct StepBuilder {
     int nreg = 0;
-    int regbit[kG];        // register t -> chunk-local bit
-    int last[kG];          // last slot used per register bit
+    int maxreg = kG;       // register bits this step may take (kG5: a 32x32 dense step)
+    int regbit[kG5];       // register t -> chunk-local bit
+    int last[kG5];         // last slot used per register bit
     StepDev sd;
     std::vector<std::pair<int, int>> placed;  // (slot * 2 + sub, op index)
     bool dense = false;    // the step applies one fused 16x16 matrix
     bool has_probe = false;
     Cd factor = {1.0, 0.0};  // scalar taken out of this step's slot gates
     double cost = 0.0;     // estimated multiply-adds per amplitude of the slot form
-    Cd M[256];             // dense matrix (row-major), valid when dense
+    Cd M[1024];            // dense matrix (row-major, dim() x dim()), valid when dense
 
     StepBuilder()
     {
         std::memset(&sd, 0, sizeof(sd));
         sd.dense = -1;
-        for (int t = 0; t < kG; ++t) last[t] = -1, regbit[t] = -1;
+        sd.bit4 = -1;
+        for (int t = 0; t < kG5; ++t) last[t] = -1, regbit[t] = -1;
         for (int t = 0; t < k1qSlots; ++t) sd.probe[t] = -1;
     }
+    int dim() const { return 1 << maxreg; }
     int reg_of(int chunk_bit) const
     {
         for (int t = 0; t < nreg; ++t)
@@ -472,49 +477,41 @@ int place(Ctx &c, StepBuilder &sb, int idx, int rx, int ry)
     return 1;
 }
 
-// 16x16 embedding of op `idx` on register bits reg[0..k-1] (its qubits in
-// matrix-index order): out = E * in.
-void embed(Ctx &c, int idx, const int *reg, Cd *E)
+// Fold op `idx` (its qubits on register bits reg[0..k-1], in matrix-index
+// order) into the dense matrix of the step: M <- U M, U acting on the rows'
+// bits reg[] (rows with those bits zero taken 2^k at a time).
+void fold_dense(Ctx &c, StepBuilder &sb, int idx, const int *reg)
 {
     const PlanOp &op = c.ops[idx];
     int q[4];
     const int k = op_qubits(op, q);
-    for (int i = 0; i < 256; ++i) E[i] = {0.0, 0.0};
+    const int D = sb.dim(), K1 = 1 << k;
     int mask = 0;
     for (int j = 0; j < k; ++j) mask |= 1 << reg[j];
-    for (int col = 0; col < 16; ++col) {
-        int ci = 0;
-        for (int j = 0; j < k; ++j) ci |= ((col >> reg[j]) & 1) << j;
-        for (int ri = 0; ri < (1 << k); ++ri) {
-            int row = col & ~mask;
-            for (int j = 0; j < k; ++j) row |= ((ri >> j) & 1) << reg[j];
-            E[16 * row + col] = mat_at(c.mats, c.eb, (int64_t)op.mat_off + (1 << k) * ri + ci);
+    Cd U[256];
+    for (int i = 0; i < K1 * K1; ++i) U[i] = mat_at(c.mats, c.eb, (int64_t)op.mat_off + i);
+    int rows[16];
+    Cd tmp[16];
+    for (int base = 0; base < D; ++base) {
+        if (base & mask) continue;
+        for (int ri = 0; ri < K1; ++ri) {
+            int r = base;
+            for (int j = 0; j < k; ++j) r |= ((ri >> j) & 1) << reg[j];
+            rows[ri] = r;
+        }
+        for (int col = 0; col < D; ++col) {
+            for (int ri = 0; ri < K1; ++ri) {
+                double re = 0, im = 0;
+                for (int ci = 0; ci < K1; ++ci) {
+                    const Cd u = U[K1 * ri + ci], m = sb.M[D * rows[ci] + col];
+                    re += u.re * m.re - u.im * m.im;
+                    im += u.re * m.im + u.im * m.re;
+                }
+                tmp[ri] = {re, im};
+            }
+            for (int ri = 0; ri < K1; ++ri) sb.M[D * rows[ri] + col] = tmp[ri];
         }
     }
-}
-
-void matmul16(const Cd *A, const Cd *B, Cd *out)  // out = A * B
-{
-    for (int r = 0; r < 16; ++r)
-        for (int q = 0; q < 16; ++q) {
-            double re = 0, im = 0;
-            for (int k = 0; k < 16; ++k) {
-                const Cd a = A[16 * r + k], b = B[16 * k + q];
-                re += a.re * b.re - a.im * b.im;
-                im += a.re * b.im + a.im * b.re;
-            }
-            out[16 * r + q] = {re, im};
-        }
-}
-
-// Fold op `idx` (its qubits on register bits reg[]) into the dense matrix of
-// the step.
-void fold_dense(Ctx &c, StepBuilder &sb, int idx, const int *reg)
-{
-    Cd E[256], T[256];
-    embed(c, idx, reg, E);
-    matmul16(E, sb.M, T);
-    std::memcpy(sb.M, T, sizeof(T));
     sb.placed.push_back({(int)sb.placed.size(), idx});
 }
 
@@ -523,7 +520,8 @@ void to_dense(Ctx &c, StepBuilder &sb)
 {
     std::vector<std::pair<int, int>> seq = sb.placed;
     std::sort(seq.begin(), seq.end());
-    for (int k = 0; k < 256; ++k) sb.M[k] = {(k % 17) == 0 ? 1.0 : 0.0, 0.0};
+    const int D = sb.dim();
+    for (int k = 0; k < D * D; ++k) sb.M[k] = {(k % (D + 1)) == 0 ? 1.0 : 0.0, 0.0};
     sb.placed.clear();
     sb.dense = true;
     sb.factor = {1.0, 0.0};
@@ -671,8 +669,10 @@ bool plan_passes(int n, const PlanOp *ops, int nops, const void *mats, int elem_
         std::vector<int> srem = chosen;
         while (!srem.empty()) {
             StepBuilder sb;
-            // fused plans: every step is one 16x16 product of all its ops
-            if (fuse == 1 && !any_probe) to_dense(ctx, sb);
+            // fused plans: every step is one 16x16 (fuse 3: 32x32 over five
+            // register bits) product of all its ops
+            if (fuse == 3 && !any_probe && !generic && elem_bytes == 8 && K >= kG5) sb.maxreg = kG5;
+            if ((fuse == 1 || fuse == 3) && !any_probe) to_dense(ctx, sb);
             uint64_t sblk = 0;
             std::vector<int> sleft;
             for (int idx : srem) {
@@ -696,7 +696,7 @@ bool plan_passes(int n, const PlanOp *ops, int nops, const void *mats, int elem_
                 for (int j = 0; j < k && ok; ++j) {
                     reg[j] = sb.reg_of(cb[j]);
                     if (reg[j] < 0) {
-                        if (sb.nreg < kG) {
+                        if (sb.nreg < sb.maxreg) {
                             reg[j] = sb.nreg;
                             sb.regbit[sb.nreg++] = cb[j];
                         } else {
@@ -730,15 +730,16 @@ bool plan_passes(int n, const PlanOp *ops, int nops, const void *mats, int elem_
             // a slot step whose arithmetic exceeds one dense product becomes dense
             if (!generic && !sb.dense && !sb.has_probe && sb.cost > (fuse == 2 ? 32.0 : 64.0)) to_dense(ctx, sb);
             // fill unused register bits with the lowest free chunk bits
-            for (int b = 0; b < K && sb.nreg < kG; ++b)
+            for (int b = 0; b < K && sb.nreg < sb.maxreg; ++b)
                 if (sb.reg_of(b) < 0) sb.regbit[sb.nreg++] = b;
             for (int t = 0; t < kG; ++t) sb.sd.bits[t] = sb.regbit[t];
+            if (sb.maxreg == kG5) sb.sd.bit4 = sb.regbit[kG];
             int sorted_bits[kG];
             std::memcpy(sorted_bits, sb.regbit, sizeof(sorted_bits));
             std::sort(sorted_bits, sorted_bits + kG);
             for (int t = 0; t < kG; ++t) sb.sd.sbits[t] = sorted_bits[t];
             if (sb.dense) {
-                sb.sd.dense = push_mats(out, sb.M, 256);
+                sb.sd.dense = push_mats(out, sb.M, sb.dim() * sb.dim());
                 sb.sd.nrounds = 0;
             } else {
                 int used = -1;
@@ -785,7 +786,7 @@ bool plan_passes(int n, const PlanOp *ops, int nops, const void *mats, int elem_
         };
         for (int si = pd.step_off; si < pd.step_off + pd.nsteps; ++si) {
             const StepDev &sd = out.steps[si];
-            if (sd.dense >= 0) cover(sd.dense, 256);
+            if (sd.dense >= 0) cover(sd.dense, sd.bit4 >= 0 ? 1024 : 256);
             if (sd.scale >= 0) cover(sd.scale, 1);
             for (int k = 0; k < kSlots; ++k)
                 if (sd.kind[k] != S1_SKIP) cover(sd.mat[k], k % kSlotsPerRound < 4 ? 8 : 4);

@@ -105,7 +105,9 @@ struct Plan {
 // product's 64 multiply-adds per amplitude on the vector units; 1, every
 // register step becomes dense; 2, as 0 with the bar at 32 multiply-adds, the
 // cost of the product on the FP32 matrix cores (48 real multiply-adds per
-// amplitude at twice the vector rate).
+// amplitude at twice the vector rate); 3 (complex64, not generic), every
+// register step becomes one dense 32x32 matrix over five register bits
+// (StepDev::bit4 holds the fifth), else as 1.
 // in_map (n entries, or null): the input is in another layout (see
 // Plan::in_map); the first pass also holds the images of the lowest cmin
 // input bit positions, so it reads runs of 2^cmin contiguous amplitudes.

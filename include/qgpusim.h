@@ -233,7 +233,12 @@ int qsim_choose_layout(int num_qubits, int dtype, int cmin, const int32_t *perm_
  *    units (64 multiply-adds per amplitude);
  * 1: every register step is fused;
  * 2: a step is fused when its gates cost more than the product on the matrix
- *    cores (32 vector multiply-adds per amplitude). */
+ *    cores (32 vector multiply-adds per amplitude);
+ * 3: every register step is fused over five register bits: one 5-qubit
+ *    (32x32) unitary per step, two 16-row halves of eight 16x16x4 k-blocks
+ *    each on the matrix cores.  Specialised complex64 kernels only: with the
+ *    run-time compiler off (qsim_set_jit) or a complex128 state, mode 3 runs
+ *    as mode 1. */
 int qsim_set_fusion(qsim_state *st, int mode);
 
 /* No reference counterpart (its custom backend has no sampling task):

@@ -679,19 +679,29 @@ def test_batch_rejects_bad_offsets():
 
 
 @pytest.mark.parametrize("dtype,K", [(np.complex64, 12), (np.complex64, 9), (np.complex64, 13), (np.complex128, 11)])
-@pytest.mark.parametrize("circ", ["random", "qft", "mixed"])
-@pytest.mark.parametrize("mode", [1, 2])
+@pytest.mark.parametrize("circ", ["random", "qft", "mixed", "multi"])
+@pytest.mark.parametrize("mode", [1, 2, 3])
 def test_fused_steps_match_oracle(hip, dtype, K, circ, mode):
     """Fusion mode 1: every register step is one 16x16 unitary (the product of
     its gates); mode 2: only the steps whose gates cost more than that product
-    on the matrix cores.  complex64 specialised kernels apply fused steps on
-    the matrix cores (3M FP32 MFMA), the slot kernel and complex128 by the
-    vector units."""
+    on the matrix cores; mode 3: every step one 32x32 unitary over five
+    register bits (complex64 specialised kernels; else as mode 1).  complex64
+    specialised kernels apply fused steps on the matrix cores (3M FP32 MFMA),
+    the slot kernel and complex128 by the vector units.  "multi" adds 3- and
+    4-qubit unitaries, which a 5-bit step fuses with their neighbours."""
     n = 14
     if circ == "random":
         ops = _tuples(C.random_circuit(n, 12, 77 + K))
     elif circ == "qft":
         ops = _tuples(C.qft_circuit(n))
+    elif circ == "multi":
+        rng = np.random.default_rng(40 + K)
+        ops = _mixed_ops(n, 5 + K)[:60]
+        for _ in range(8):
+            for k in (3, 4):
+                qs = [int(x) for x in rng.choice(n, k, replace=False)]
+                ops.append((f"{k}q", *qs, _haar(k, rng)))
+            ops.append(("1q", int(rng.integers(n)), G.rx(rng.uniform(0, 6))))
     else:
         ops = _mixed_ops(n, 5 + K)
     psi0 = C.random_state(n, 6)

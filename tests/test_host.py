@@ -872,8 +872,9 @@ def test_specialised_pass_kernels_compile(dtype, K, cmin):
     npass = hip.jit_compile_check(n, arr, len(ops), arena, K, cmin)
     passes, _ = hip.plan_describe(n, arr, len(ops), arena, K, cmin)
     assert npass == len(passes) > 0
-    # fused steps (matrix-core GEMMs for complex64) compile too
+    # fused steps (matrix-core GEMMs for complex64, 16x16 and 32x32) compile too
     assert hip.jit_compile_check(n, arr, len(ops), arena, K, cmin, fuse=True) > 0
+    assert hip.jit_compile_check(n, arr, len(ops), arena, K, cmin, fuse=3) > 0
 
 
 @pytest.mark.parametrize("use_float32", [True, False])
