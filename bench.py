@@ -501,9 +501,35 @@ def fused_legs(hip, sv, prog, args, bytes_per_pass, ref_state):
     """Both fusion modes; ref_state (the plain path's state) or None."""
     out = {"fused": fused_leg(hip, sv, prog, args, bytes_per_pass, ref_state, 1),
            "fused_selective": fused_leg(hip, sv, prog, args, bytes_per_pass, ref_state, 2),
-           "fused5": fused_leg(hip, sv, prog, args, bytes_per_pass, ref_state, 3)}
+           "fused5": fused_leg(hip, sv, prog, args, bytes_per_pass, ref_state, 3),
+           "fused5_selective": fused_leg(hip, sv, prog, args, bytes_per_pass, ref_state, 4)}
+    out["fusion_sweep"] = fusion_sweep(hip, sv, prog, args)
     sv.set_fusion(0)
+    sv.set_fusion_bars(0, 0)
     return out
+
+
+def fusion_sweep(hip, sv, prog, args):
+    """Fused steps chosen by measured cost: the circuit timed under each cost
+    bar of modes 2 (a 4-qubit step fused when its slot arithmetic exceeds
+    `bar` multiply-adds per amplitude) and 4 (a 5-qubit step kept when its
+    gates' vector work exceeds `bar5`), beside mode 0; the fastest is named."""
+    cands = [(0, 0, 0)] + [(2, b, 0) for b in (16, 24, 32, 48)] + [(4, 32, b5) for b5 in (32, 48, 64, 96, 128)]
+    rows = []
+    for mode, bar, bar5 in cands:
+        sv.set_fusion(mode)
+        sv.set_fusion_bars(bar, bar5)
+        run_program(hip, sv, prog, hip.GRAPHED)
+        sv.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            run_program(hip, sv, prog, hip.GRAPHED)
+        sv.synchronize()
+        rows.append({"mode": mode, "bar": bar, "bar5": bar5,
+                     "s_per_circuit": round((time.perf_counter() - t0) / args.steps, 4)})
+        log(f"fusion sweep: mode {mode} bar {bar} bar5 {bar5}: {rows[-1]['s_per_circuit']:.4f} s/circuit")
+    best = min(rows, key=lambda r: r["s_per_circuit"])
+    return {"rows": rows, "best": best}
 
 
 def fused_leg(hip, sv, prog, args, bytes_per_pass, ref_state, mode):

@@ -92,6 +92,7 @@ _SIGNATURES = {
     "qsim_jit_compile_check": ([_I, _I, _P, _I, _P, _U64, _I, _I, _I, _P, _P, _P], _I),
     "qsim_choose_layout": ([_I, _I, _I, _P, _P, _I, _P], _I),
     "qsim_set_fusion": ([_P, _I], _I),
+    "qsim_set_fusion_bars": ([_P, ctypes.c_double, ctypes.c_double], _I),
     "qsim_device_memory": ([_P, _P], _I),
     "qsim_sample": ([_P, _U64, _U64, _P], _I),
     "qsim_sample_noalloc": ([_P, _U64, _U64, _P], _I),
@@ -426,8 +427,15 @@ class _StatevectorBase:
         4-qubit unitary on the matrix cores (complex64, specialised kernels);
         2: a step fused when its gates cost more than that matrix-core product;
         3: every register step one fused 5-qubit (32x32) unitary on the matrix
-        cores (complex64 with specialised kernels; otherwise as mode 1)."""
+        cores (complex64 with specialised kernels; otherwise as mode 1);
+        4: as 2, with a 5-qubit step wherever the gates it takes cost more
+        than bar5 on the vector units (set_fusion_bars)."""
         _check(lib.qsim_set_fusion(self._h, int(mode)))
+
+    def set_fusion_bars(self, bar=0.0, bar5=0.0):
+        """Cost bars of fusion modes 2 and 4 in real multiply-adds per
+        amplitude (0: defaults 32 and 64)."""
+        _check(lib.qsim_set_fusion_bars(self._h, float(bar), float(bar5)))
 
     def set_jit(self, enable=True):
         """Run forward chunk passes with kernels specialised per pass (hiprtc,

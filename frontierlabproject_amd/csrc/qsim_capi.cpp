@@ -148,6 +148,7 @@ struct qsim_state {
     bool zero_pending = false;
     bool jit = true;               // specialise forward chunk passes at run time
     int fuse = 0;                  // fusion mode (qsim_set_fusion)
+    double fuse_bar = 0.0, fuse_bar5 = 0.0;  // its cost bars (qsim_set_fusion_bars; 0: defaults)
     std::string arch;              // device gcnArchName (for run-time compiles)
     // per-kernel-class timing with events on `stream` (off by default)
     bool prof = false, capturing = false;
@@ -445,7 +446,7 @@ int plan_for(qsim_state *st, const qsim_op *ops, int nops, const void *mats, int
     const int K = max_probes > 1 ? std::min(st->K, st->dtype == QSIM_C64 ? 12 : 11) : st->K;
     // kernels compiled ahead of time take the generic slot forms
     if (!plan_passes(st->n, pops.data(), nops, mats, (int)amp_bytes(st->dtype), K, st->cmin, max_probes, plan,
-                     err, max_probes == 1 ? st->fuse : 0, in_map, out_map, !st->jit))
+                     err, max_probes == 1 ? st->fuse : 0, in_map, out_map, !st->jit, st->fuse_bar, st->fuse_bar5))
         return fail(err);
     return 0;
 }
@@ -695,6 +696,8 @@ int run_graphed(qsim_state *st, const qsim_op *ops, int nops, const void *mats, 
     put(&st->cmin, sizeof(int));
     put(&st->jit, sizeof(bool));
     put(&st->fuse, sizeof(int));
+    put(&st->fuse_bar, sizeof(double));
+    put(&st->fuse_bar5, sizeof(double));
     const bool keep = can_fold(st);
     put(&keep, sizeof(bool));
     if (keep) put(st->perm_map, sizeof(int32_t) * st->n);
@@ -1468,8 +1471,18 @@ int qsim_set_jit(qsim_state *st, int enable)
 int qsim_set_fusion(qsim_state *st, int mode)
 {
     if (!st) return fail("null state");
-    if (mode < 0 || mode > 3) return fail("fusion mode must be 0, 1, 2 or 3");
+    if (mode < 0 || mode > 4) return fail("fusion mode must be between 0 and 4");
     st->fuse = mode;
+    return 0;
+}
+
+int qsim_set_fusion_bars(qsim_state *st, double bar, double bar5)
+{
+    if (!st) return fail("null state");
+    if (!(bar >= 0.0 && bar <= 4096.0) || !(bar5 >= 0.0 && bar5 <= 4096.0))
+        return fail("fusion bars must be between 0 and 4096");
+    st->fuse_bar = bar;
+    st->fuse_bar5 = bar5;
     return 0;
 }
 

@@ -207,7 +207,9 @@ int next_slot(int last, int target)
 
 constexpr int kG5 = kG + 1;  // register bits of a 32x32 dense step
 
-struct StepBuilder {
+// Everything of a step under construction but its dense matrix (an op that
+// does not fit leaves the matrix untouched, so a retry saves only this).
+struct StepCore {
     int nreg = 0;
     int maxreg = kG;       // register bits this step may take (kG5: a 32x32 dense step)
     int regbit[kG5];       // register t -> chunk-local bit
@@ -218,9 +220,8 @@ struct StepBuilder {
     bool has_probe = false;
     Cd factor = {1.0, 0.0};  // scalar taken out of this step's slot gates
     double cost = 0.0;     // estimated multiply-adds per amplitude of the slot form
-    Cd M[1024];            // dense matrix (row-major, dim() x dim()), valid when dense
 
-    StepBuilder()
+    StepCore()
     {
         std::memset(&sd, 0, sizeof(sd));
         sd.dense = -1;
@@ -235,6 +236,10 @@ struct StepBuilder {
             if (regbit[t] == chunk_bit) return t;
         return -1;
     }
+};
+
+struct StepBuilder : StepCore {
+    Cd M[1024];            // dense matrix (row-major, dim() x dim()), valid when dense
 };
 
 struct Ctx {
@@ -515,6 +520,21 @@ void fold_dense(Ctx &c, StepBuilder &sb, int idx, const int *reg)
     sb.placed.push_back({(int)sb.placed.size(), idx});
 }
 
+// Vector arithmetic an op costs on its own, in real multiply-adds per
+// amplitude: four per nonzero entry of an average matrix row.
+double op_work(const Ctx &c, int idx)
+{
+    const PlanOp &op = c.ops[idx];
+    int q[4];
+    const int D = 1 << op_qubits(op, q);
+    int nnz = 0;
+    for (int i = 0; i < D * D; ++i) {
+        const Cd m = mat_at(c.mats, c.eb, (int64_t)op.mat_off + i);
+        nnz += m.re != 0.0 || m.im != 0.0;
+    }
+    return 4.0 * nnz / D;
+}
+
 // Turn a slot step into a dense one: multiply its placed ops in slot order.
 void to_dense(Ctx &c, StepBuilder &sb)
 {
@@ -539,10 +559,13 @@ void to_dense(Ctx &c, StepBuilder &sb)
 
 bool plan_passes(int n, const PlanOp *ops, int nops, const void *mats, int elem_bytes, int K, int cmin,
                  int max_probes, Plan &out, std::string &err, int fuse, const int32_t *in_map,
-                 const int32_t *out_map, bool generic)
+                 const int32_t *out_map, bool generic, double bar, double bar5)
 {
     out = Plan();
     out.generic = generic;
+    // slot steps costlier than bar4 multiply-adds per amplitude become dense
+    const double bar4 = (fuse == 2 || fuse == 4) ? (bar > 0.0 ? bar : kFuseBar) : 64.0;
+    if (bar5 <= 0.0) bar5 = kFuseBar5;
     if (n < kG || n > 63) {
         err = "tile plan: state must have between 4 and 63 qubits";
         return false;
@@ -666,15 +689,11 @@ bool plan_passes(int n, const PlanOp *ops, int nops, const void *mats, int elem_
         Ctx ctx{ops, mats, elem_bytes, &out, &pd, max_probes, loc, !any_probe && !generic, generic};
 
         // ---- register steps ----
-        std::vector<int> srem = chosen;
-        while (!srem.empty()) {
-            StepBuilder sb;
-            // fused plans: every step is one 16x16 (fuse 3: 32x32 over five
-            // register bits) product of all its ops
-            if (fuse == 3 && !any_probe && !generic && elem_bytes == 8 && K >= kG5) sb.maxreg = kG5;
-            if ((fuse == 1 || fuse == 3) && !any_probe) to_dense(ctx, sb);
+        // grow: place ops of srem into the step in order (an op whose bits
+        // meet a skipped op's waits); the rest go to sleft.  work sums the
+        // placed ops' own vector cost.
+        auto grow = [&](StepBuilder &sb, const std::vector<int> &srem, std::vector<int> &sleft, double *work) {
             uint64_t sblk = 0;
-            std::vector<int> sleft;
             for (int idx : srem) {
                 const PlanOp &op = ops[idx];
                 int q[4], cb[4], reg[4];
@@ -689,7 +708,7 @@ bool plan_passes(int n, const PlanOp *ops, int nops, const void *mats, int elem_
                     sleft.push_back(idx);
                     continue;
                 }
-                StepBuilder save = sb;
+                const StepCore save = sb;
                 const size_t mats_before = out.mats.size(), ids_before = out.probe_ids.size();
                 const int np_before = pd.nprobe;
                 bool ok = true;
@@ -718,8 +737,9 @@ bool plan_passes(int n, const PlanOp *ops, int nops, const void *mats, int elem_
                         }
                     }
                 }
+                if (ok && work) *work += op_work(ctx, idx);
                 if (!ok) {
-                    sb = save;
+                    static_cast<StepCore &>(sb) = save;
                     out.mats.resize(mats_before);
                     out.probe_ids.resize(ids_before);
                     pd.nprobe = np_before;
@@ -727,8 +747,36 @@ bool plan_passes(int n, const PlanOp *ops, int nops, const void *mats, int elem_
                     sleft.push_back(idx);
                 }
             }
+        };
+        std::vector<int> srem = chosen;
+        while (!srem.empty()) {
+            StepBuilder sb;
+            std::vector<int> sleft;
+            bool five = false;
+            if (fuse == 4 && !any_probe && !generic && elem_bytes == 8 && K >= kG5) {
+                // one 32x32 step over five register bits when the vector work
+                // of the ops it takes exceeds bar5
+                StepBuilder b5;
+                b5.maxreg = kG5;
+                to_dense(ctx, b5);
+                double work = 0.0;
+                grow(b5, srem, sleft, &work);
+                if (work > bar5) {
+                    sb = b5;
+                    five = true;
+                } else {
+                    sleft.clear();
+                }
+            }
+            if (!five) {
+                // fused plans: every step is one 16x16 (fuse 3: 32x32 over
+                // five register bits) product of all its ops
+                if (fuse == 3 && !any_probe && !generic && elem_bytes == 8 && K >= kG5) sb.maxreg = kG5;
+                if ((fuse == 1 || fuse == 3) && !any_probe) to_dense(ctx, sb);
+                grow(sb, srem, sleft, nullptr);
+            }
             // a slot step whose arithmetic exceeds one dense product becomes dense
-            if (!generic && !sb.dense && !sb.has_probe && sb.cost > (fuse == 2 ? 32.0 : 64.0)) to_dense(ctx, sb);
+            if (!generic && !sb.dense && !sb.has_probe && sb.cost > bar4) to_dense(ctx, sb);
             // fill unused register bits with the lowest free chunk bits
             for (int b = 0; b < K && sb.nreg < sb.maxreg; ++b)
                 if (sb.reg_of(b) < 0) sb.regbit[sb.nreg++] = b;

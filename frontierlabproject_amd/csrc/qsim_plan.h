@@ -107,7 +107,8 @@ struct Plan {
 // cost of the product on the FP32 matrix cores (48 real multiply-adds per
 // amplitude at twice the vector rate); 3 (complex64, not generic), every
 // register step becomes one dense 32x32 matrix over five register bits
-// (StepDev::bit4 holds the fifth), else as 1.
+// (StepDev::bit4 holds the fifth), else as 1; 4, as 2 with 32x32 steps
+// where they take enough work (below; complex64, not generic).
 // in_map (n entries, or null): the input is in another layout (see
 // Plan::in_map); the first pass also holds the images of the lowest cmin
 // input bit positions, so it reads runs of 2^cmin contiguous amplitudes.
@@ -116,9 +117,16 @@ struct Plan {
 // generic: only the generic slot forms, no scalars taken out, and no step
 // made dense for its cost (plans for the compiled slot-program kernels, where
 // the kinds are only known at run time and code size is what limits speed).
+// bar (fuse 2 and 4; 0: kFuseBar): the slot cost above which a 4-bit step
+// becomes dense.  bar5 (fuse 4; 0: kFuseBar5): a step is first grown as one
+// 32x32 matrix over five register bits, and kept when the vector work of the
+// ops it takes (four real multiply-adds per nonzero of an average matrix
+// row, summed) exceeds bar5; else it is built as for fuse 2.
+constexpr double kFuseBar = 32.0, kFuseBar5 = 64.0;
 bool plan_passes(int n, const PlanOp *ops, int nops, const void *mats, int elem_bytes, int K,
                  int cmin, int max_probes, Plan &out, std::string &err, int fuse = 0,
-                 const int32_t *in_map = nullptr, const int32_t *out_map = nullptr, bool generic = false);
+                 const int32_t *in_map = nullptr, const int32_t *out_map = nullptr, bool generic = false,
+                 double bar = 0.0, double bar5 = 0.0);
 
 // Pauli chunk passes: each term's X support must lie inside its pass's chunk.
 struct PauliPass {

@@ -238,8 +238,19 @@ int qsim_choose_layout(int num_qubits, int dtype, int cmin, const int32_t *perm_
  *    (32x32) unitary per step, two 16-row halves of eight 16x16x4 k-blocks
  *    each on the matrix cores.  Specialised complex64 kernels only: with the
  *    run-time compiler off (qsim_set_jit) or a complex128 state, mode 3 runs
- *    as mode 1. */
+ *    as mode 1;
+ * 4: as 2, except that each step is first grown as one 32x32 unitary over
+ *    five register bits and kept when the ops it takes would cost more than
+ *    bar5 (qsim_set_fusion_bars) on the vector units; complex64 specialised
+ *    kernels only, else as 2. */
 int qsim_set_fusion(qsim_state *st, int mode);
+
+/* No reference counterpart.  Cost bars of fusion modes 2 and 4, in real
+ * multiply-adds per amplitude (0: the defaults, 32 and 64): bar, the slot
+ * arithmetic above which a 4-qubit step is fused; bar5, the vector work of
+ * the ops (four per nonzero of an average matrix row, summed over the ops)
+ * above which mode 4 keeps a 32x32 step.  Each in [0, 4096]. */
+int qsim_set_fusion_bars(qsim_state *st, double bar, double bar5);
 
 /* No reference counterpart (its custom backend has no sampling task):
  * `shots` computational-basis outcomes drawn from |psi|^2 / <psi|psi>, as

@@ -680,12 +680,14 @@ def test_batch_rejects_bad_offsets():
 
 @pytest.mark.parametrize("dtype,K", [(np.complex64, 12), (np.complex64, 9), (np.complex64, 13), (np.complex128, 11)])
 @pytest.mark.parametrize("circ", ["random", "qft", "mixed", "multi"])
-@pytest.mark.parametrize("mode", [1, 2, 3])
+@pytest.mark.parametrize("mode", [1, 2, 3, 4])
 def test_fused_steps_match_oracle(hip, dtype, K, circ, mode):
     """Fusion mode 1: every register step is one 16x16 unitary (the product of
     its gates); mode 2: only the steps whose gates cost more than that product
     on the matrix cores; mode 3: every step one 32x32 unitary over five
-    register bits (complex64 specialised kernels; else as mode 1).  complex64
+    register bits (complex64 specialised kernels; else as mode 1); mode 4:
+    mode 2 plus 32x32 steps where their gates' vector work exceeds bar5 (set
+    low here, 16, so that both step sizes occur).  complex64
     specialised kernels apply fused steps on the matrix cores (3M FP32 MFMA),
     the slot kernel and complex128 by the vector units.  "multi" adds 3- and
     4-qubit unitaries, which a 5-bit step fuses with their neighbours."""
@@ -713,6 +715,8 @@ def test_fused_steps_match_oracle(hip, dtype, K, circ, mode):
         sv = _sv(hip, n, dtype)
         sv.set_tile_params(K, 4 if dtype == np.complex64 else 3)
         sv.set_fusion(mode)
+        if mode == 4:
+            sv.set_fusion_bars(24, 16)
         sv.set_jit(jit)
         sv.from_numpy(psi0)
         sv.apply_tile_cooperative(ops)
@@ -720,6 +724,17 @@ def test_fused_steps_match_oracle(hip, dtype, K, circ, mode):
         _check(ref, got, dtype)
         outs.append(got)
     assert dense.fidelity(outs[0], outs[1]) > 1 - TOL[dtype]
+
+
+def test_fusion_settings_refused(hip):
+    sv = hip.StatevectorF32(8)
+    for bad in (-1, 5):
+        with pytest.raises(RuntimeError, match="fusion mode must be between 0 and 4"):
+            sv.set_fusion(bad)
+    for bars in ((-1, 0), (0, 5000), (float("nan"), 0)):
+        with pytest.raises(RuntimeError, match="fusion bars must be between 0 and 4096"):
+            sv.set_fusion_bars(*bars)
+    sv.set_fusion_bars(0, 0)
 
 
 def _layouts(n, rng, k):

@@ -875,6 +875,7 @@ def test_specialised_pass_kernels_compile(dtype, K, cmin):
     # fused steps (matrix-core GEMMs for complex64, 16x16 and 32x32) compile too
     assert hip.jit_compile_check(n, arr, len(ops), arena, K, cmin, fuse=True) > 0
     assert hip.jit_compile_check(n, arr, len(ops), arena, K, cmin, fuse=3) > 0
+    assert hip.jit_compile_check(n, arr, len(ops), arena, K, cmin, fuse=4) > 0
 
 
 @pytest.mark.parametrize("use_float32", [True, False])
