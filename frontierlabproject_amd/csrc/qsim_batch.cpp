@@ -93,6 +93,7 @@ std::vector<PlanOp> plan_ops(const qsim_op *ops, int64_t n)
 int append_plan(const Plan &p, std::vector<PassDev> &passes, std::vector<StepDev> &steps,
                 std::vector<int32_t> &ids, std::vector<cx<float>> &mats)
 {
+    if (!p.fdiag.empty()) return -1;  // free diagonals need specialised kernels
     const int pass0 = (int)passes.size();
     const int step0 = (int)steps.size(), id0 = (int)ids.size(), mat0 = (int)mats.size();
     for (PassDev d : p.passes) {
@@ -261,6 +262,7 @@ int qsim_batch_load(qsim_batch *b, int ncirc, const int32_t *num_qubits, const u
         C.fwd0 = append_plan(pf[i], passes, steps, ids, cmats);
         C.nfwd = (int)pf[i].passes.size();
         C.rev0 = append_plan(pr[i], passes, steps, ids, cmats);
+        if (C.fwd0 < 0 || C.rev0 < 0) return set_error("internal: batch plan with free diagonals");
         C.nrev = (int)pr[i].passes.size();
         C.probe0 = (int32_t)probe0;
         C.nprobe = nprobes[i];

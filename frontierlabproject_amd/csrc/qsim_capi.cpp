@@ -420,9 +420,12 @@ int enqueue_passes(qsim_state *st, const Plan &plan, const DescImage &img, char 
                                           1, 1, 0, st->stream, lam ? args2 : args1, nullptr));
             st->counters[5]++;
         } else {
-            // 32x32 dense steps exist only in specialised kernels
-            for (int si = hp.step_off; si < hp.step_off + hp.nsteps; ++si)
-                if (plan.steps[si].bit4 >= 0) return fail("internal: 32x32 dense step without its specialised kernel");
+            // 32x32 dense steps and free diagonals exist only in specialised kernels
+            for (int si = hp.step_off; si < hp.step_off + hp.nsteps; ++si) {
+                const FreeDiag *fd = nullptr;
+                if (plan.steps[si].bit4 >= 0 || plan.step_fdiag(si, &fd) > 0)
+                    return fail("internal: step form without its specialised kernel");
+            }
             HIP_TRY(launch_tile_pass<R>(static_cast<cx<R> *>(st->psi), static_cast<cx<R> *>(lam), st->n, hp, dp + p,
                                         ds, dm, st->d_probe, plan.generic, st->stream));
         }

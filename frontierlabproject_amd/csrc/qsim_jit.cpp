@@ -44,7 +44,73 @@ bool skip_pass(const Plan &plan, size_t p, int nv) { return nv == 1 && plan.pass
 constexpr int kPairF[6] = {0, 2, 0, 1, 0, 1};
 constexpr int kPairS[6] = {1, 3, 2, 3, 3, 2};
 
-void emit_step(std::ostringstream &o, const StepDev &sd, int E, const char *T, int nv, int tpb)
+// The free diagonals of a step, after its slot rounds, with the step's
+// factor: each thread's group bits outside the registers (gcb, the chunk
+// index of its register index 0) select every factor; a diagonal on a
+// register bit t and an outside bit gives the pair (pt0, pt1) applied by
+// that register bit's value.  The amplitudes then take the products from a
+// table over the register bits used.
+void emit_free_diagonals(std::ostringstream &o, const StepDev &sd, const FreeDiag *fd, int nfd, const char *T,
+                         int nv)
+{
+    auto reg = [&](int bit) {
+        for (int t = 0; t < 4; ++t)
+            if (sd.bits[t] == bit) return t;
+        return -1;
+    };
+    auto bitv = [](int bit) { return "((gcb >> " + std::to_string(bit) + ") & 1u)"; };
+    auto mat = [](int off) { return "mats[" + std::to_string(off) + "]"; };
+    const std::string C = std::string("cx<") + T + ">";
+    o << "    { const uint32_t gcb = ins0(ins0(ins0(ins0(g32, " << sd.sbits[0] << "), " << sd.sbits[1] << "), "
+      << sd.sbits[2] << "), " << sd.sbits[3] << ");\n";
+    o << "      " << C << " ff = " << (sd.scale >= 0 ? mat(sd.scale) : C + "{1, 0}") << ";\n";
+    bool used[4] = {false, false, false, false};
+    for (int f = 0; f < nfd; ++f) {
+        const int ra = reg(fd[f].a), rb = fd[f].b >= 0 ? reg(fd[f].b) : -1;
+        if (ra >= 0) used[ra] = true;
+        if (rb >= 0) used[rb] = true;
+    }
+    for (int t = 0; t < 4; ++t)
+        if (used[t]) o << "      " << C << " p" << t << "0 = {1, 0}, p" << t << "1 = {1, 0};\n";
+    for (int f = 0; f < nfd; ++f) {
+        const int mo = fd[f].mo, a = fd[f].a, b = fd[f].b;
+        const int ra = reg(a), rb = b >= 0 ? reg(b) : -1;
+        if (b < 0) {  // 1q, its bit outside the registers
+            o << "      ff = cmul(ff, " << bitv(a) << " ? " << mat(mo + 1) << " : " << mat(mo) << ");\n";
+        } else if (ra < 0 && rb < 0) {
+            o << "      { const uint32_t c = " << bitv(a) << " | (" << bitv(b) << " << 1);\n        ff = cmul(ff, (c & 2u) ? "
+              << "((c & 1u) ? " << mat(mo + 3) << " : " << mat(mo + 2) << ") : ((c & 1u) ? " << mat(mo + 1) << " : "
+              << mat(mo) << ")); }\n";
+        } else if (ra >= 0) {  // matrix index m_a + 2 g_b
+            o << "      { const bool g = " << bitv(b) << ";\n        p" << ra << "0 = cmul(p" << ra << "0, g ? "
+              << mat(mo + 2) << " : " << mat(mo) << "); p" << ra << "1 = cmul(p" << ra << "1, g ? " << mat(mo + 3)
+              << " : " << mat(mo + 1) << "); }\n";
+        } else {  // matrix index g_a + 2 m_b
+            o << "      { const bool g = " << bitv(a) << ";\n        p" << rb << "0 = cmul(p" << rb << "0, g ? "
+              << mat(mo + 1) << " : " << mat(mo) << "); p" << rb << "1 = cmul(p" << rb << "1, g ? " << mat(mo + 3)
+              << " : " << mat(mo + 2) << "); }\n";
+        }
+    }
+    int U[4], nu = 0;
+    for (int t = 0; t < 4; ++t)
+        if (used[t]) U[nu++] = t;
+    o << "      " << C << " tb[" << (1 << nu) << "];\n      tb[0] = ff;\n";
+    for (int i = 0; i < nu; ++i)
+        for (int e = 0; e < (1 << i); ++e)
+            o << "      tb[" << e + (1 << i) << "] = cmul(tb[" << e << "], p" << U[i] << "1); tb[" << e << "] = cmul(tb["
+              << e << "], p" << U[i] << "0);\n";
+    const char *vec[2] = {"a", "l"};
+    for (int v = 0; v < nv; ++v)
+        for (int m = 0; m < 16; ++m) {
+            int e = 0;
+            for (int i = 0; i < nu; ++i) e |= ((m >> U[i]) & 1) << i;
+            o << "      " << vec[v] << "[" << m << "] = cmul(" << vec[v] << "[" << m << "], tb[" << e << "]);\n";
+        }
+    o << "    }\n";
+}
+
+void emit_step(std::ostringstream &o, const StepDev &sd, int E, const char *T, int nv, int tpb,
+               const FreeDiag *fd = nullptr, int nfd = 0)
 {
     const int NG = E >> kG;
     int moff[16];
@@ -152,7 +218,9 @@ void emit_step(std::ostringstream &o, const StepDev &sd, int E, const char *T, i
                       << ", mats + " << sd.mat[b + 4 + j] << ");\n";
             }
         }
-        if (sd.scale >= 0) {
+        if (nfd > 0) {
+            emit_free_diagonals(o, sd, fd, nfd, T, nv);
+        } else if (sd.scale >= 0) {
             o << "    { const cx<" << T << "> f = mats[" << sd.scale << "];\n";
             for (int v = 0; v < nv; ++v)
                 o << "      #pragma unroll\n      for (int m = 0; m < 16; ++m) " << vec[v] << "[m] = cmul(" << vec[v]
@@ -166,6 +234,13 @@ void emit_step(std::ostringstream &o, const StepDev &sd, int E, const char *T, i
     }
     if (loop) o << "  }\n";
     o << "  }\n  __syncthreads();\n";
+}
+
+void emit_plan_step(std::ostringstream &o, const Plan &plan, int s, int E, const char *T, int nv, int tpb)
+{
+    const FreeDiag *fd = nullptr;
+    const int nfd = plan.step_fdiag(s, &fd);
+    emit_step(o, plan.steps[s], E, T, nv, tpb, fd, nfd);
 }
 
 std::string kernel_name(const std::string &body) { return "qsim_jit_" + std::to_string(fnv1a(body)); }
@@ -477,7 +552,7 @@ std::string remapped_pass_source(const Plan &plan, size_t p, bool dbl, const std
     b << "  const uint64_t base_out = 0ull" << bit_moves("c", Cout, pd.ncomp, "uint64_t") << ";\n";
     emit_copy(b, ci, "i", "psi", "c0", "base_in", Pin, Jin, K, true, dbl);
     b << "  __syncthreads();\n";
-    for (int s = 0; s < pd.nsteps; ++s) emit_step(b, plan.steps[pd.step_off + s], E, T, 1, TPB);
+    for (int s = 0; s < pd.nsteps; ++s) emit_plan_step(b, plan, pd.step_off + s, E, T, 1, TPB);
     emit_copy(b, co, "o", "out", "c0", "base_out", Pout, Jout, K, false, dbl);
     return lane_table_source(lt, name) + o.str() + b.str() + "}\n\n";
 }
@@ -525,7 +600,7 @@ std::string mapped_pass_source(const Plan &plan, bool dbl, const std::string &na
         o << "  chunk_copy_in_mapped<" << TPB << ">(psi, c0, base_in, rowin, " << E << ", " << CFR << ", sigma, "
           << J[0] << ");\n";
     o << "  __syncthreads();\n";
-    for (int s = 0; s < pd.nsteps; ++s) emit_step(o, plan.steps[pd.step_off + s], E, T, 1, TPB);
+    for (int s = 0; s < pd.nsteps; ++s) emit_plan_step(o, plan, pd.step_off + s, E, T, 1, TPB);
     o << "  chunk_copy<" << T << ", false, true, " << TPB << ">(out, c0, base, rowoff, " << E << ", " << CF
       << ");\n}\n\n";
     return o.str();
@@ -601,7 +676,7 @@ std::string jit_pass_source(const Plan &plan, size_t p, bool dbl, int nv, const 
         }
         if (nv == 2) emit_copy(b, cp, "", "lam", "c1", "base", pd.pos, ident, K, true, dbl);
         b << "  __syncthreads();\n";
-        for (int s = 0; s < pd.nsteps; ++s) emit_step(b, plan.steps[pd.step_off + s], E, T, nv, TPB);
+        for (int s = 0; s < pd.nsteps; ++s) emit_plan_step(b, plan, pd.step_off + s, E, T, nv, TPB);
         emit_copy(b, cp, "", "psi", "c0", "base", pd.pos, ident, K, false, dbl);
         if (nv == 2) {
             emit_copy(b, cp, "", "lam", "c1", "base", pd.pos, ident, K, false, dbl);
@@ -643,7 +718,7 @@ std::string jit_pass_source(const Plan &plan, size_t p, bool dbl, int nv, const 
     }
     if (nv == 2) o << "  chunk_copy<" << T << ", true" << ntf << ">(lam, c1, base, rowoff, " << E << ", " << CF << ");\n";
     o << "  __syncthreads();\n";
-    for (int s = 0; s < pd.nsteps; ++s) emit_step(o, plan.steps[pd.step_off + s], E, T, nv, TPB);
+    for (int s = 0; s < pd.nsteps; ++s) emit_plan_step(o, plan, pd.step_off + s, E, T, nv, TPB);
     o << "  chunk_copy<" << T << ", false" << ntf << ">(psi, c0, base, rowoff, " << E << ", " << CF << ");\n";
     if (nv == 2) {
         o << "  chunk_copy<" << T << ", false" << ntf << ">(lam, c1, base, rowoff, " << E << ", " << CF << ");\n";

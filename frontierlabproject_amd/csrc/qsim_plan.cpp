@@ -220,6 +220,9 @@ struct StepCore {
     bool has_probe = false;
     Cd factor = {1.0, 0.0};  // scalar taken out of this step's slot gates
     double cost = 0.0;     // estimated multiply-adds per amplitude of the slot form
+    uint64_t lock = 0;     // chunk bits that stay outside the registers (free diagonals)
+    uint64_t endbits = 0;  // chunk bits of free diagonals: no later non-diagonal gate here
+    std::vector<FreeDiag> fd;  // free diagonals, applied after the slot rounds
 
     StepCore()
     {
@@ -252,6 +255,7 @@ struct Ctx {
     const int *loc;      // state bit -> chunk-local bit
     bool real_scale;     // non-unit scalars may be factored out (no probes)
     bool generic;        // generic slot forms only (S1_GEN, S1_DIAG; S2_CPHASE, S2_CU_*)
+    const std::vector<uint8_t> *diag;  // per op: a diagonal 1q/2q gate that may be a free diagonal
 };
 
 int push_mats(Plan &p, const Cd *v, int count)
@@ -520,6 +524,47 @@ void fold_dense(Ctx &c, StepBuilder &sb, int idx, const int *reg)
     sb.placed.push_back({(int)sb.placed.size(), idx});
 }
 
+// Place diagonal op `idx` (chunk bits cb[0..k-1]) as a free diagonal of the
+// step: at least one of its bits is not a register bit; those become locked
+// out of the registers, provided enough unlocked bits remain to fill them.
+bool place_free(Ctx &c, StepBuilder &sb, int idx, const int *cb, int k)
+{
+    int outside = 0;
+    uint64_t lock = sb.lock, m = 0;
+    for (int j = 0; j < k; ++j) {
+        m |= 1ull << cb[j];
+        if (sb.reg_of(cb[j]) < 0) {
+            ++outside;
+            lock |= 1ull << cb[j];
+        }
+    }
+    if (outside == 0) return false;
+    int free_bits = 0;  // chunk bits neither registers nor locked
+    for (int b = 0; b < c.pd->K; ++b) free_bits += !((lock >> b) & 1) && sb.reg_of(b) < 0;
+    if (free_bits < sb.maxreg - sb.nreg) return false;
+    const PlanOp &op = c.ops[idx];
+    const int D = 1 << k;
+    Cd d[4];
+    for (int i = 0; i < D; ++i) d[i] = mat_at(c.mats, c.eb, (int64_t)op.mat_off + (D + 1) * i);
+    sb.lock = lock;
+    sb.endbits |= m;
+    sb.fd.push_back({cb[0], k > 1 ? cb[1] : -1, push_mats(*c.plan, d, D)});
+    sb.placed.push_back({1000 + (int)sb.fd.size(), idx});  // after every slot
+    sb.cost += 1.0;
+    return true;
+}
+
+// A 1q or 2q op whose matrix is diagonal.
+bool diagonal_op(const PlanOp &op, const void *mats, int eb)
+{
+    if (op.kind != OP_1Q && op.kind != OP_2Q) return false;
+    const int D = op.kind == OP_1Q ? 2 : 4;
+    for (int r = 0; r < D; ++r)
+        for (int q = 0; q < D; ++q)
+            if (r != q && !is0(mat_at(mats, eb, (int64_t)op.mat_off + D * r + q))) return false;
+    return true;
+}
+
 // Vector arithmetic an op costs on its own, in real multiply-adds per
 // amplitude: four per nonzero entry of an average matrix row.
 double op_work(const Ctx &c, int idx)
@@ -615,6 +660,11 @@ bool plan_passes(int n, const PlanOp *ops, int nops, const void *mats, int elem_
     const uint64_t low = cmin ? ((1ull << cmin) - 1) : 0;
     bool any_probe = false;
     for (int i = 0; i < nops; ++i) any_probe |= ops[i].kind == OP_PROBE;
+    // diagonal gates may become free diagonals (specialised one-vector plans)
+    std::vector<uint8_t> diag(nops, 0);
+    if (!generic && !any_probe)
+        for (int i = 0; i < nops; ++i) diag[i] = diagonal_op(ops[i], mats, elem_bytes);
+    out.fd_off.push_back(0);
     Cd pending = {1.0, 0.0};  // scalar factored out of finished steps, not yet applied
 
     if (in_map) {
@@ -686,7 +736,7 @@ bool plan_passes(int n, const PlanOp *ops, int nops, const void *mats, int elem_
         pd.step_off = (int32_t)out.steps.size();
         pd.probe_off = (int32_t)out.probe_ids.size();
         pd.nprobe = 0;
-        Ctx ctx{ops, mats, elem_bytes, &out, &pd, max_probes, loc, !any_probe && !generic, generic};
+        Ctx ctx{ops, mats, elem_bytes, &out, &pd, max_probes, loc, !any_probe && !generic, generic, &diag};
 
         // ---- register steps ----
         // grow: place ops of srem into the step in order (an op whose bits
@@ -703,7 +753,9 @@ bool plan_passes(int n, const PlanOp *ops, int nops, const void *mats, int elem_
                     cb[j] = loc[q[j]];
                     m |= 1ull << cb[j];
                 }
-                if (m & sblk) {
+                // a non-diagonal gate waits for the free diagonals on its bits
+                const bool dg = diag[idx] != 0;
+                if ((m & sblk) || (!dg && (m & sb.endbits))) {
                     sblk |= m;
                     sleft.push_back(idx);
                     continue;
@@ -715,7 +767,7 @@ bool plan_passes(int n, const PlanOp *ops, int nops, const void *mats, int elem_
                 for (int j = 0; j < k && ok; ++j) {
                     reg[j] = sb.reg_of(cb[j]);
                     if (reg[j] < 0) {
-                        if (sb.nreg < sb.maxreg) {
+                        if (sb.nreg < sb.maxreg && !((sb.lock >> cb[j]) & 1)) {
                             reg[j] = sb.nreg;
                             sb.regbit[sb.nreg++] = cb[j];
                         } else {
@@ -729,13 +781,19 @@ bool plan_passes(int n, const PlanOp *ops, int nops, const void *mats, int elem_
                         if (ok) fold_dense(ctx, sb, idx, reg);
                     } else {
                         const int r = place(ctx, sb, idx, reg[0], k > 1 ? reg[1] : -1);
-                        if (r == 2 && !sb.has_probe) {
+                        if (r == 2 && !sb.has_probe && sb.fd.empty()) {
                             to_dense(ctx, sb);
                             fold_dense(ctx, sb, idx, reg);
                         } else {
                             ok = r == 1;
                         }
                     }
+                }
+                if (!ok && dg && !sb.dense) {
+                    // a diagonal gate that found no register room or slot
+                    static_cast<StepCore &>(sb) = save;
+                    out.mats.resize(mats_before);
+                    ok = place_free(ctx, sb, idx, cb, k);
                 }
                 if (ok && work) *work += op_work(ctx, idx);
                 if (!ok) {
@@ -776,10 +834,10 @@ bool plan_passes(int n, const PlanOp *ops, int nops, const void *mats, int elem_
                 grow(sb, srem, sleft, nullptr);
             }
             // a slot step whose arithmetic exceeds one dense product becomes dense
-            if (!generic && !sb.dense && !sb.has_probe && sb.cost > bar4) to_dense(ctx, sb);
+            if (!generic && !sb.dense && !sb.has_probe && sb.fd.empty() && sb.cost > bar4) to_dense(ctx, sb);
             // fill unused register bits with the lowest free chunk bits
-            for (int b = 0; b < K && sb.nreg < sb.maxreg; ++b)
-                if (sb.reg_of(b) < 0) sb.regbit[sb.nreg++] = b;
+            for (int b = 0; b < pd.K && sb.nreg < sb.maxreg; ++b)
+                if (sb.reg_of(b) < 0 && !((sb.lock >> b) & 1)) sb.regbit[sb.nreg++] = b;
             for (int t = 0; t < kG; ++t) sb.sd.bits[t] = sb.regbit[t];
             if (sb.maxreg == kG5) sb.sd.bit4 = sb.regbit[kG];
             int sorted_bits[kG];
@@ -806,6 +864,8 @@ bool plan_passes(int n, const PlanOp *ops, int nops, const void *mats, int elem_
             std::sort(sb.placed.begin(), sb.placed.end());
             for (auto &pr : sb.placed) out.order.push_back(pr.second);
             out.steps.push_back(sb.sd);
+            out.fdiag.insert(out.fdiag.end(), sb.fd.begin(), sb.fd.end());
+            out.fd_off.push_back((int32_t)out.fdiag.size());
             srem.swap(sleft);
         }
         pd.nsteps = (int32_t)out.steps.size() - pd.step_off;
@@ -813,6 +873,7 @@ bool plan_passes(int n, const PlanOp *ops, int nops, const void *mats, int elem_
         rem.swap(left);
     }
     if (out_folded) out.out_map.assign(out_map, out_map + n);
+    if (out.fdiag.empty()) out.fd_off.clear();
     // the remaining factor goes to the last step
     if (!(pending.re == 1.0 && pending.im == 0.0) && !out.steps.empty()) {
         StepDev &last = out.steps.back();
@@ -836,6 +897,9 @@ bool plan_passes(int n, const PlanOp *ops, int nops, const void *mats, int elem_
             const StepDev &sd = out.steps[si];
             if (sd.dense >= 0) cover(sd.dense, sd.bit4 >= 0 ? 1024 : 256);
             if (sd.scale >= 0) cover(sd.scale, 1);
+            const FreeDiag *fd = nullptr;
+            const int nfd = out.step_fdiag(si, &fd);
+            for (int f = 0; f < nfd; ++f) cover(fd[f].mo, fd[f].b >= 0 ? 4 : 2);
             for (int k = 0; k < kSlots; ++k)
                 if (sd.kind[k] != S1_SKIP) cover(sd.mat[k], k % kSlotsPerRound < 4 ? 8 : 4);
         }

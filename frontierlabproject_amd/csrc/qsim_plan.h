@@ -66,6 +66,17 @@ inline uint64_t op_mask(const PlanOp &op)
     return m;
 }
 
+// A diagonal gate applied after its register step's slot rounds, with at
+// least one qubit on a chunk bit outside the step's register bits: each
+// thread multiplies its amplitudes by phase factors chosen by its group's
+// bits (specialised kernels only).  Its bits outside the registers stay
+// outside them for the rest of the step, and no later non-diagonal gate of
+// the step touches its bits, so applying it last keeps the gate order.
+struct FreeDiag {
+    int32_t a, b;  // chunk-local bits of the op's qubits in matrix-index order (b = -1: 1q)
+    int32_t mo;    // arena offset of its diagonal (2 or 4 complex entries)
+};
+
 struct Plan {
     std::vector<PassDev> passes;
     std::vector<StepDev> steps;
@@ -89,6 +100,16 @@ struct Plan {
     // Only the generic slot forms (see S2_CU_FIRST): the plan runs on the
     // compiled slot-program kernels.
     bool generic = false;
+    // Free diagonals of step s: fdiag[fd_off[s] .. fd_off[s + 1]); both
+    // empty when the plan has none.
+    std::vector<FreeDiag> fdiag;
+    std::vector<int32_t> fd_off;
+    int step_fdiag(int s, const FreeDiag **fd) const
+    {
+        if (fd_off.empty()) return 0;
+        *fd = fdiag.data() + fd_off[s];
+        return fd_off[s + 1] - fd_off[s];
+    }
 };
 
 // Scalar factors (a diagonal gate's first entry, H's 1/sqrt2) are taken out of

@@ -726,6 +726,72 @@ def test_fused_steps_match_oracle(hip, dtype, K, circ, mode):
     assert dense.fidelity(outs[0], outs[1]) > 1 - TOL[dtype]
 
 
+def _diagonal_heavy_ops(n, seed):
+    """H / RX / CX layers between runs of diagonal gates on random qubit
+    pairs and single qubits: CP, CZ, RZZ (a diagonal with four distinct
+    entries), a random diagonal 4x4, RZ, T, S."""
+    rng = np.random.default_rng(seed)
+    ops = []
+    for layer in range(4):
+        for q in rng.choice(n, 3, replace=False):
+            ops.append(("1q", int(q), G.h() if layer % 2 == 0 else G.rx(rng.uniform(0, 6))))
+        for _ in range(25):
+            a, b = (int(x) for x in rng.choice(n, 2, replace=False))
+            th = rng.uniform(0, 6)
+            kind = int(rng.integers(0, 6))
+            if kind == 0:
+                ops.append(("2q", a, b, G.cp(th)))
+            elif kind == 1:
+                ops.append(("2q", a, b, G.cz()))
+            elif kind == 2:
+                ops.append(("2q", a, b, np.diag(np.exp(-0.5j * th * np.array([1, -1, -1, 1])))))
+            elif kind == 3:
+                ops.append(("2q", a, b, np.diag(np.exp(1j * rng.uniform(0, 6, 4)))))
+            elif kind == 4:
+                ops.append(("1q", a, [G.rz(th), G.t(), G.s()][int(rng.integers(0, 3))]))
+            else:
+                ops.append(("2q", a, b, G.cx()))
+    return ops
+
+
+@pytest.mark.parametrize("dtype,K", [(np.complex64, 13), (np.complex64, 9), (np.complex64, 6),
+                                     (np.complex128, 12), (np.complex128, 7)])
+@pytest.mark.parametrize("circ", ["diag", "qft"])
+def test_free_diagonals_match_oracle(hip, dtype, K, circ):
+    """Diagonal gates with a qubit outside a register step's four register
+    bits run as per-thread phase factors after the step (specialised kernels
+    only): against the FP64 oracle, in the cooperative and graphed modes,
+    from the logical order and after a pending layout change (the folded
+    first pass), and against the slot-program kernel, which has none."""
+    n = 14
+    ops = _diagonal_heavy_ops(n, 50 + K) if circ == "diag" else _tuples(C.qft_circuit(n))
+    psi0 = C.random_state(n, 8)
+    rng = np.random.default_rng(K)
+    lay = [int(x) for x in rng.permutation(n)]
+    ref = psi0
+    for o in ops:
+        ref = dense.apply(ref, o[-1], list(o[1:-1]), n)
+    ref_p = P.permute(psi0, list(range(n)), lay)
+    for o in ops:
+        ref_p = dense.apply(ref_p, o[-1], list(o[1:-1]), n)
+    outs = {}
+    for jit in (True, False):
+        for mode in ("apply_tile_cooperative", "apply_tile_graphed"):
+            sv = _sv(hip, n, dtype)
+            sv.set_tile_params(K, min(4 if dtype == np.complex64 else 3, K - 2))
+            sv.set_jit(jit)
+            sv.from_numpy(psi0)
+            getattr(sv, mode)(ops)
+            got = sv.to_numpy()
+            _check(ref, got, dtype)
+            outs[jit, mode] = got
+            sv.from_numpy(psi0)
+            sv.permute(list(range(n)), lay)
+            getattr(sv, mode)(ops)
+            _check(ref_p, sv.to_numpy(), dtype)
+    assert dense.fidelity(outs[True, "apply_tile_graphed"], outs[False, "apply_tile_graphed"]) > 1 - TOL[dtype]
+
+
 def test_fusion_settings_refused(hip):
     sv = hip.StatevectorF32(8)
     for bad in (-1, 5):

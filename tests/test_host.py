@@ -853,6 +853,36 @@ def test_chunk_plan_keeps_probe_order(hip):
     assert sorted(order) == list(range(n))
 
 
+def test_free_diagonals_keep_steps_few():
+    """Diagonal gates whose qubits are not all register bits become per-thread
+    phase factors of the current step instead of new register steps: 4 H then
+    40 controlled phases over 12 chunk bits, twice, plan as at most 4 steps
+    (one per H group, each holding its phases), where gate-by-gate register
+    steps need one per three phases.  Every op is planned exactly once, and an
+    H never runs before a phase on its qubit that precedes it."""
+    from frontierlabproject_amd.backends.custom.native import qgpusim_hip as hip
+    n = 12
+    rng = np.random.default_rng(3)
+    ops = []
+    for half in range(2):
+        for q in range(4 * half, 4 * half + 4):
+            ops.append(("1q", q, G.h()))
+        for _ in range(40):
+            a, b = (int(x) for x in rng.choice(n, 2, replace=False))
+            ops.append(("2q", a, b, G.cp(rng.uniform(0, 6))))
+    arr, arena = hip.pack_ops(ops, n, np.complex64)
+    passes, order = hip.plan_describe(n, arr, len(ops), arena, 12, 4, steps=True)
+    assert sorted(order) == list(range(len(ops)))
+    assert len(passes) == 1 and passes[0][2] <= 4, passes[0][2]
+    pos = {i: order.index(i) for i in range(len(ops))}
+    for i, o in enumerate(ops):
+        if o[0] == "1q":
+            for j in range(i):
+                if o[1] in ops[j][1:-1]:
+                    assert pos[j] < pos[i], (j, i)
+    assert hip.jit_compile_check(n, arr, len(ops), arena, 12, 4) == 1
+
+
 @pytest.mark.parametrize("dtype,K,cmin", [(np.complex64, 12, 4), (np.complex64, 13, 4), (np.complex128, 11, 3),
                                          (np.complex128, 12, 3), (np.complex64, 7, 2)])
 def test_specialised_pass_kernels_compile(dtype, K, cmin):
