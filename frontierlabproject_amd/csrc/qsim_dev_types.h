@@ -51,6 +51,39 @@ QSIM_HD inline constexpr int swz(int e)
 }
 // Slot of the single element 2^k.
 QSIM_HD inline constexpr int swz_bit(int k) { return (1 << k) ^ swz_pattern(k); }
+// complex128 chunks: one amplitude per 16-byte slot, so a bank line of 128
+// (256) bytes holds 8 (16) slots and a wave's 16-byte accesses spread over
+// the banks when its lanes' slots differ in bits 0..2 (0..3).  A register
+// step's lanes differ in the lowest non-register chunk bits; these patterns
+// (bits 3..13 into bits 0..3, chosen by a search over every 4-bit register
+// set of a 12-bit chunk) keep those bits' images independent for most sets:
+// the average lane-group multiplicity drops from 1.75 (1.46) to 1.19 (1.19).
+// Measured on the 33-qubit complex128 circuit: LDS bank-conflict cycles from
+// 28.9 % to 24.8 % of the chunk passes' time, passes 55.2 -> 54.3 ms.
+QSIM_HD inline constexpr int swz_pattern_d(int k)
+{
+    return k == 3 ? 0x3 : k == 4 ? 0x7 : k == 5 ? 0xe : k == 6 ? 0xd : k == 7 ? 0x8 : k == 8 ? 0xd : k == 9 ? 0x3
+         : k == 10 ? 0x5 : k == 11 ? 0x9 : k == 12 ? 0x3 : k == 13 ? 0xa : 0;
+}
+QSIM_HD inline constexpr int swz_d(int e)
+{
+    int s = e;
+    for (int k = 3; k <= 13; ++k)
+        if ((e >> k) & 1) s ^= swz_pattern_d(k);
+    return s;
+}
+// The swizzle of a chunk of cx<R>: every kernel templated on the precision
+// addresses its LDS through this one.
+template <typename R>
+QSIM_HD inline constexpr int swzr(int e)
+{
+    return sizeof(R) == 8 ? swz_d(e) : swz(e);
+}
+template <typename R>
+QSIM_HD inline constexpr int swzr_bit(int k)
+{
+    return swzr<R>(1 << k);
+}
 constexpr int kMaxComp = 50;   // bits outside a chunk
 #ifndef QSIM_ROUNDS
 #define QSIM_ROUNDS 4

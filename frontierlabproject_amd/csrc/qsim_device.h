@@ -544,7 +544,7 @@ __device__ __forceinline__ void chunk_copy(cx<R> *__restrict__ g, cx<R> *__restr
     const int lowmask = (1 << cf) - 1;
     if (sizeof(cx<R>) == 8 && cf >= 1) {
         const int nv = E >> 1;
-        const int st = swz((int)threadIdx.x << 1);
+        const int st = swzr<R>((int)threadIdx.x << 1);
         for (int v0 = 0; v0 < nv; v0 += TPB * kBatch) {
             v4f buf[kBatch];
             if (LOAD) {
@@ -557,13 +557,13 @@ __device__ __forceinline__ void chunk_copy(cx<R> *__restrict__ g, cx<R> *__restr
 #pragma unroll
                 for (int j = 0; j < kBatch; ++j) {
                     const int v = v0 + j * TPB + threadIdx.x;
-                    if (v < nv) *reinterpret_cast<v4f *>(lds + (st ^ swz((v0 + j * TPB) << 1))) = buf[j];
+                    if (v < nv) *reinterpret_cast<v4f *>(lds + (st ^ swzr<R>((v0 + j * TPB) << 1))) = buf[j];
                 }
             } else {
 #pragma unroll
                 for (int j = 0; j < kBatch; ++j) {
                     const int v = v0 + j * TPB + threadIdx.x;
-                    if (v < nv) buf[j] = *reinterpret_cast<const v4f *>(lds + (st ^ swz((v0 + j * TPB) << 1)));
+                    if (v < nv) buf[j] = *reinterpret_cast<const v4f *>(lds + (st ^ swzr<R>((v0 + j * TPB) << 1)));
                 }
 #pragma unroll
                 for (int j = 0; j < kBatch; ++j) {
@@ -578,7 +578,7 @@ __device__ __forceinline__ void chunk_copy(cx<R> *__restrict__ g, cx<R> *__restr
         const raw *gr = reinterpret_cast<const raw *>(g);
         raw *gw = reinterpret_cast<raw *>(g);
         raw *lr = reinterpret_cast<raw *>(lds);
-        const int st = swz((int)threadIdx.x);
+        const int st = swzr<R>((int)threadIdx.x);
         for (int e0 = 0; e0 < E; e0 += TPB * kBatch) {
             raw buf[kBatch];
             if (LOAD) {
@@ -590,13 +590,13 @@ __device__ __forceinline__ void chunk_copy(cx<R> *__restrict__ g, cx<R> *__restr
 #pragma unroll
                 for (int j = 0; j < kBatch; ++j) {
                     const int e = e0 + j * TPB + threadIdx.x;
-                    if (e < E) lr[st ^ swz(e0 + j * TPB)] = buf[j];
+                    if (e < E) lr[st ^ swzr<R>(e0 + j * TPB)] = buf[j];
                 }
             } else {
 #pragma unroll
                 for (int j = 0; j < kBatch; ++j) {
                     const int e = e0 + j * TPB + threadIdx.x;
-                    if (e < E) buf[j] = lr[st ^ swz(e0 + j * TPB)];
+                    if (e < E) buf[j] = lr[st ^ swzr<R>(e0 + j * TPB)];
                 }
 #pragma unroll
                 for (int j = 0; j < kBatch; ++j) {
@@ -663,7 +663,7 @@ __device__ __forceinline__ void chunk_copy_in_mapped_d(const cx<double> *__restr
     const raw *gr = reinterpret_cast<const raw *>(g);
     raw *lr = reinterpret_cast<raw *>(lds);
     const int lowmask = (1 << cf) - 1;
-    const int st = swz(sigma((int)threadIdx.x));
+    const int st = swz_d(sigma((int)threadIdx.x));
     for (int r0 = 0; r0 < E; r0 += TPB * kBatch) {
         raw buf[kBatch];
 #pragma unroll
@@ -674,7 +674,7 @@ __device__ __forceinline__ void chunk_copy_in_mapped_d(const cx<double> *__restr
 #pragma unroll
         for (int j = 0; j < kBatch; ++j) {
             const int r = r0 + j * TPB + threadIdx.x;
-            if (r < E) lr[st ^ swz(sigma(r0 + j * TPB))] = buf[j];
+            if (r < E) lr[st ^ swz_d(sigma(r0 + j * TPB))] = buf[j];
         }
     }
 }
@@ -697,11 +697,11 @@ __device__ __forceinline__ void run_steps(cx<R> *c0, cx<R> *c1, PP pd, SP steps,
         const int s0 = sd->sbits[0], s1 = sd->sbits[1], s2 = sd->sbits[2], s3 = sd->sbits[3];
         const int nr = sd->nrounds, dn = sd->dense, sc = sd->scale;
         // swz is XOR-linear and the group base and register offsets occupy
-        // disjoint bits, so each element's LDS slot is swz(base) ^ swz(offset):
+        // disjoint bits, so each element's LDS slot is swzr<R>(base) ^ swzr<R>(offset):
         // one XOR per element with a wave-uniform constant.
         // (the 16 offsets are XOR combinations of the four register bits' slots)
         int moff[16];
-        const int b0 = swz_bit(r0), b1 = swz_bit(r1), b2 = swz_bit(r2), b3 = swz_bit(r3);
+        const int b0 = swzr_bit<R>(r0), b1 = swzr_bit<R>(r1), b2 = swzr_bit<R>(r2), b3 = swzr_bit<R>(r3);
 #pragma unroll
         for (int m = 0; m < 16; ++m)
             moff[m] = ((m & 1) ? b0 : 0) ^ ((m & 2) ? b1 : 0) ^ ((m & 4) ? b2 : 0) ^ ((m & 8) ? b3 : 0);
@@ -719,7 +719,7 @@ __device__ __forceinline__ void run_steps(cx<R> *c0, cx<R> *c1, PP pd, SP steps,
             const bool act = gi < ngroups;
             // idle lanes read a valid group and never store
             const uint32_t g32 = (uint32_t)(act ? gi : ngroups - 1);
-            const int sb = swz((int)ins0(ins0(ins0(ins0(g32, s0), s1), s2), s3));
+            const int sb = swzr<R>((int)ins0(ins0(ins0(ins0(g32, s0), s1), s2), s3));
             cx<R> a[16], l[NV == 2 ? 16 : 1];
 #pragma unroll
             for (int m = 0; m < 16; ++m) {

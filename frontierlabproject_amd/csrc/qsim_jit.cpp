@@ -25,7 +25,7 @@ std::map<std::pair<int, std::string>, hipFunction_t> g_fns;  // (device, source)
 std::vector<hipModule_t> g_modules;                           // never unloaded
 std::atomic<long> g_compiled{0};
 
-int swz_host(int e) { return swz(e); }
+int swz_host(int e, bool dbl) { return dbl ? swz_d(e) : swz(e); }
 
 uint64_t fnv1a(const std::string &s)
 {
@@ -113,10 +113,12 @@ void emit_step(std::ostringstream &o, const StepDev &sd, int E, const char *T, i
                const FreeDiag *fd = nullptr, int nfd = 0)
 {
     const int NG = E >> kG;
+    const bool dbl = std::string(T) == "double";
     int moff[16];
     for (int m = 0; m < 16; ++m)
         moff[m] = swz_host(((m & 1) << sd.bits[0]) | (((m >> 1) & 1) << sd.bits[1]) |
-                           (((m >> 2) & 1) << sd.bits[2]) | (((m >> 3) & 1) << sd.bits[3]));
+                               (((m >> 2) & 1) << sd.bits[2]) | (((m >> 3) & 1) << sd.bits[3]),
+                           dbl);
     const bool loop = NG > tpb, guard = NG < tpb;
     if (sd.bit4 >= 0) {  // 32x32 on the matrix cores (complex64 one-vector passes only)
         int bits5[5] = {sd.bits[0], sd.bits[1], sd.bits[2], sd.bits[3], sd.bit4};
@@ -127,7 +129,7 @@ void emit_step(std::ostringstream &o, const StepDev &sd, int E, const char *T, i
         for (int m = 0; m < 32; ++m) {
             int off = 0;
             for (int t = 0; t < 5; ++t) off |= ((m >> t) & 1) << bits5[t];
-            o << swz_host(off) << (m < 31 ? ", " : "");
+            o << swz_host(off, false) << (m < 31 ? ", " : "");
         }
         o << "};\n    dense32_mfma<" << tpb << ">(c0, mats + " << sd.dense << ", moff";
         for (int t = 0; t < 5; ++t) o << ", " << sorted5[t];
@@ -155,15 +157,15 @@ void emit_step(std::ostringstream &o, const StepDev &sd, int E, const char *T, i
           << ");\n";
     else
         o << "    const uint32_t g32 = (uint32_t)gi;\n";
-    // with chunk bit 0 among the register bits the base slot is even (swz
-    // keeps bit 0); saying so lets the compiler pair slots sb, sb ^ 1 into
-    // 16-byte LDS accesses
+    // complex64: with chunk bit 0 among the register bits the base slot is
+    // even (swz keeps bit 0); saying so lets the compiler pair slots sb, sb ^ 1
+    // into 16-byte LDS accesses.  (The complex128 swizzle feeds bit 0.)
     bool bit0 = false;
-    for (int q = 0; q < 4; ++q) bit0 |= sd.bits[q] == 0;
+    for (int q = 0; q < 4; ++q) bit0 |= sd.bits[q] == 0 && !dbl;
     // LDS addressing in bytes (slot * amplitude size): the xor with each
     // register offset then needs no shift
     const int esz = std::string(T) == "float" ? 8 : 16;
-    o << "    const int sb = (swz((int)ins0(ins0(ins0(ins0(g32, " << sd.sbits[0] << "), " << sd.sbits[1] << "), "
+    o << "    const int sb = (swzr<" << T << ">((int)ins0(ins0(ins0(ins0(g32, " << sd.sbits[0] << "), " << sd.sbits[1] << "), "
       << sd.sbits[2] << "), " << sd.sbits[3] << "))" << (bit0 ? " & ~1" : "") << ") * " << esz << ";\n";
     for (int m = 0; m < 16; ++m) moff[m] *= esz;
     // vectors: a (psi, in c0) and, with nv == 2, l (lam, in c1)
@@ -407,7 +409,7 @@ CopyPlan copy_plan(LaneTable &lt, int K, bool dbl, int tpb, const int *src, cons
     for (uint32_t l = 0; l < 64; ++l) {
         const uint64_t r = (uint64_t)l * c.per;
         off[l] = (int32_t)(deposit(r, src, c.lb) * esz);
-        slot[l] = swz_host((int)deposit(r, chunk, c.lb)) * esz;  // LDS bytes
+        slot[l] = swz_host((int)deposit(r, chunk, c.lb), dbl) * esz;  // LDS bytes
     }
     c.row = lt.add_row(off);
     lt.add_row(slot);
@@ -423,7 +425,8 @@ void emit_copy_thread(std::ostringstream &o, const CopyPlan &c, const char *tag,
     o << "  const uint32_t Tl" << t << " = (uint32_t)qtab[" << c.row * 64 << " + lane];\n";
     o << "  const uint32_t rw" << t << " = wv << " << c.lb << ";\n";
     o << "  const uint64_t Tw" << t << " = 0ull" << bit_moves(("rw" + t).c_str(), src, c.tb, "uint64_t") << ";\n";
-    o << "  const int st" << t << " = qtab[" << (c.row + 1) * 64 << " + lane] ^ (swz(0"
+    o << "  const int st" << t << " = qtab[" << (c.row + 1) * 64 << " + lane] ^ (" << (c.per == 1 ? "swz_d" : "swz")
+      << "(0"
       << bit_moves(("rw" + t).c_str(), chunk, c.tb, "int") << ") * " << (c.per == 2 ? 8 : 16) << ");\n";
 }
 
@@ -440,7 +443,7 @@ void emit_copy(std::ostringstream &o, const CopyPlan &c, const char *tag, const 
     bool ident = true;
     for (int k = 0; k < K; ++k) ident &= chunk[k] == k;
     const int esz = dbl ? 16 : 8;
-    const int s1 = swz_host(1 << chunk[0]) * esz;  // complex64: a pair's second amplitude (LDS bytes, xor)
+    const int s1 = swz_host(1 << chunk[0], dbl) * esz;  // complex64: a pair's second amplitude (LDS bytes, xor)
     for (int j0 = 0; j0 < c.nj; j0 += B) {
         const int nb = std::min(B, c.nj - j0);
         o << "  {\n    v4f b[" << nb << "];\n";
@@ -449,7 +452,7 @@ void emit_copy(std::ostringstream &o, const CopyPlan &c, const char *tag, const 
                 const std::string bq = "b[" + std::to_string(q) + "]";
                 const uint64_t rj = (uint64_t)(j0 + q) << c.tb;
                 const uint64_t Cj = deposit(rj, src, K);
-                const int Sj = swz_host((int)deposit(rj, chunk, K)) * esz;
+                const int Sj = swz_host((int)deposit(rj, chunk, K), dbl) * esz;
                 std::ostringstream a;
                 a << "reinterpret_cast<v4f *>(reinterpret_cast<char *>(const_cast<cx<" << T << "> *>(" << g
                   << ") + uniform_u64(" << base << " + Tw" << t << " + " << Cj << "ull)) + Tl" << t << ")";

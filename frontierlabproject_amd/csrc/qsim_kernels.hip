@@ -669,14 +669,14 @@ __global__ __launch_bounds__(kTPB) void k_pauli_chunk(const cx<R> *__restrict__ 
     __syncthreads();
 
     NSum er, ei;
-    const int st = swz((int)threadIdx.x);  // slots split like e = (e - threadIdx.x) ^ threadIdx.x
+    const int st = swzr<R>((int)threadIdx.x);  // slots split like e = (e - threadIdx.x) ^ threadIdx.x
     for (int e = threadIdx.x; e < E; e += kTPB) {
         const uint64_t addr = base + rowoff[e >> cf] + (e & lowmask);
-        const int se = swz(e - threadIdx.x) ^ st;
+        const int se = swzr<R>(e - threadIdx.x) ^ st;
         double tr = 0.0, ti = 0.0;
         for (int k = 0; k < nterms; ++k) {
             const PauliDev T = terms[k];
-            const cx<R> v = c0[se ^ swz(T.xloc)];
+            const cx<R> v = c0[se ^ swzr<R>(T.xloc)];
             const uint64_t src = addr ^ T.xmask;
             // i^{|x&z|} * (-1)^{popc(src & z)}
             const int ph = (__popcll(T.xmask & T.zmask) + 2 * __popcll(src & T.zmask)) & 3;
