@@ -1120,7 +1120,7 @@ int qsim_create(int num_qubits, int dtype, qsim_state **out)
     st->dtype = dtype;
     st->dim = 1ull << num_qubits;
     // chunk bits: 2^13 complex64 or 2^12 complex128 amplitudes (64 KB of LDS)
-    st->K = dtype == QSIM_C64 ? 13 : 12;
+    st->K = dtype == QSIM_C64 ? 13 : 11;
     st->cmin = dtype == QSIM_C64 ? 4 : 3;
     if (const char *e = std::getenv("QSIM_JIT")) st->jit = std::atoi(e) != 0;
     HIP_TRY(hipGetDevice(&st->device));

@@ -104,13 +104,15 @@ inline int far_bit(int elem_bytes) { return elem_bytes == 8 ? 18 : 17; }
 // over every set of `free` qubits among those the remaining ops touch, the
 // one after which the in-order rule needs the fewest further passes (the
 // in-order rule alone can strand an op that opens a third pass), then the one
-// with the fewest bits of `far` (bits the pass reads from far-apart rows: for
-// a folded layout change, those stored high in the old order), then the one
-// running the most ops now.  When the sets are many, only the kSearchDeep that
+// that leaves the op list's last pass room for the output order's row bits
+// `out_rows` (when an output order is to be written), then the one with the
+// fewest bits of `far` (bits the pass reads from far-apart rows: for a folded
+// layout change, those stored high in the old order), then the one running
+// the most ops now.  When the sets are many, only the kSearchDeep that
 // run the most ops now get the look-ahead.  Returns 0 when the search does not
 // apply (all qubits fit, or too many sets).
 uint64_t search_chunk(const PlanOp *ops, const std::vector<int> &rem, uint64_t P0, int Kp, uint64_t low, int K,
-                      uint64_t far)
+                      uint64_t far, uint64_t out_rows)
 {
     const int free = Kp - popc(P0);
     std::vector<uint64_t> masks;
@@ -153,7 +155,7 @@ uint64_t search_chunk(const PlanOp *ops, const std::vector<int> &rem, uint64_t P
     if (cands.size() > kSearchDeep)
         std::stable_sort(cands.begin(), cands.end(), [](const Cand &a, const Cand &b) { return a.ran > b.ran; });
     uint64_t best = 0;
-    int best_rest = 1 << 30, best_far = 1 << 30, best_ran = -1;
+    int best_rest = 1 << 30, best_far = 1 << 30, best_ran = -1, best_nofold = 2;
     const size_t deep = std::min(cands.size(), kSearchDeep);
     for (size_t k = 0; k < deep; ++k) {
         const uint64_t M = cands[k].M;
@@ -162,9 +164,23 @@ uint64_t search_chunk(const PlanOp *ops, const std::vector<int> &rem, uint64_t P
         run_ops(masks, live, M);
         const int rest = greedy_passes(masks, live, low, K);
         const int nfar = popc(M & far);
-        if (rest < best_rest || (rest == best_rest && (nfar < best_far || (nfar == best_far && ran > best_ran)))) {
+        // with an output order to write (out_rows: its row bits), prefer a
+        // last pass with room for them: this pass when nothing is left, else
+        // the next one when it is the last
+        int nofold = 0;
+        if (out_rows && rest <= 1) {
+            uint64_t U = 0;
+            for (size_t i = 0; i < masks.size(); ++i)
+                if (rest == 0 ? !live[i] : live[i]) U |= masks[i];
+            nofold = rest == 0 ? popc(P0 | U | out_rows) > Kp : popc(low | U | out_rows) > K;
+        }
+        if (rest < best_rest ||
+            (rest == best_rest &&
+             (nofold < best_nofold ||
+              (nofold == best_nofold && (nfar < best_far || (nfar == best_far && ran > best_ran)))))) {
             best = M;
             best_rest = rest;
+            best_nofold = nofold;
             best_far = nfar;
             best_ran = ran;
         }
@@ -707,7 +723,7 @@ bool plan_passes(int n, const PlanOp *ops, int nops, const void *mats, int elem_
         uint64_t far = 0;  // chunk bits this pass would gather from far-apart rows
         if (in_map && out.passes.empty())
             for (int p = far_bit(elem_bytes); p < n; ++p) far |= 1ull << in_map[p];
-        const uint64_t fixed = any_probe ? 0 : search_chunk(ops, rem, P, Kp, low, K, far);
+        const uint64_t fixed = any_probe ? 0 : search_chunk(ops, rem, P, Kp, low, K, far, out_folded ? 0 : out_rows);
         if (fixed) P = fixed;
         int nprobe = 0;
         std::vector<int> chosen, left;

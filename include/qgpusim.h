@@ -183,7 +183,7 @@ int qsim_adjoint(qsim_state *st, const qsim_pauli *terms, int nterms, const qsim
                  double *expect_out);
 
 /* Chunk-pass geometry: K chunk bits, cmin always-present low bits.  Defaults:
- * K = 13, cmin = 4 for complex64; K = 12, cmin = 3 for complex128.  Ranges:
+ * K = 13, cmin = 4 for complex64; K = 11, cmin = 3 for complex128.  Ranges:
  * 6 <= K <= 13 (12 for complex128), 0 <= cmin <= K - 2.  Two-vector adjoint
  * passes use at most K = 12 (complex64) or 11 (complex128). */
 int qsim_set_tile_params(qsim_state *st, int K, int cmin);

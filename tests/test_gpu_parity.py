@@ -959,6 +959,8 @@ def test_tile_then_layout_change_in_one_call(hip, dtype, jit, mode, n, width, de
     ref0 = P.permute(ref0, L[0], L[1])
     sv = _sv(hip, n, dtype)
     sv.set_jit(jit)
+    if dtype == np.complex128:
+        sv.set_tile_params(12, 3)  # room for one pass that both reads and writes another order
     print(f"tile_permute n={n} width={width} dtype={np.dtype(dtype).name} jit={jit} mode={mode}", flush=True)
     for rep in range(3):
         sv.from_numpy(psi0)
