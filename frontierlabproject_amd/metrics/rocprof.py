@@ -170,6 +170,18 @@ def trace_markdown(d, command, title):
         x = res[cls]
         lines.append(f"| `{cls}` | " + " | ".join(span(x[k]) for k in ("vgpr", "agpr", "sgpr", "lds", "scratch",
                                                                       "grid", "wg")) + " |")
+    # every chunk pass's duration, in launch order (one row per 16)
+    passes = []
+    with open(trace[0]) as f:
+        for r in csv.DictReader(f):
+            if r["Kernel_Name"].startswith("qsim_jit_"):
+                passes.append((int(r["Start_Timestamp"]), (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e6))
+    if passes:
+        passes.sort()
+        lines += ["", f"Chunk-pass durations in launch order, ms ({len(passes)} dispatches):", "", "```"]
+        for i in range(0, len(passes), 16):
+            lines.append(" ".join(f"{ms:6.2f}" for _, ms in passes[i:i + 16]))
+        lines.append("```")
     return "\n".join(lines) + "\n"
 
 
