@@ -55,10 +55,15 @@ void emit_free_diagonals(std::ostringstream &o, const StepDev &sd, const FreeDia
 {
     auto reg = [&](int bit) {
         for (int t = 0; t < 4; ++t)
-            if (sd.bits[t] == bit) return t;
+            if (bit >= 0 && sd.bits[t] == bit) return t;
         return -1;
     };
-    auto bitv = [](int bit) { return "((gcb >> " + std::to_string(bit) + ") & 1u)"; };
+    // a chunk bit's value from the group; a bit outside the chunk (encoded
+    // -2 - m) from bit m of the block's chunk index c
+    auto bitv = [](int bit) {
+        return bit >= 0 ? "((gcb >> " + std::to_string(bit) + ") & 1u)"
+                        : "((uint32_t)(c >> " + std::to_string(-2 - bit) + ") & 1u)";
+    };
     auto mat = [](int off) { return "mats[" + std::to_string(off) + "]"; };
     const std::string C = std::string("cx<") + T + ">";
     o << "    { const uint32_t gcb = ins0(ins0(ins0(ins0(g32, " << sd.sbits[0] << "), " << sd.sbits[1] << "), "
@@ -66,7 +71,7 @@ void emit_free_diagonals(std::ostringstream &o, const StepDev &sd, const FreeDia
     o << "      " << C << " ff = " << (sd.scale >= 0 ? mat(sd.scale) : C + "{1, 0}") << ";\n";
     bool used[4] = {false, false, false, false};
     for (int f = 0; f < nfd; ++f) {
-        const int ra = reg(fd[f].a), rb = fd[f].b >= 0 ? reg(fd[f].b) : -1;
+        const int ra = reg(fd[f].a), rb = reg(fd[f].b);
         if (ra >= 0) used[ra] = true;
         if (rb >= 0) used[rb] = true;
     }
@@ -74,12 +79,12 @@ void emit_free_diagonals(std::ostringstream &o, const StepDev &sd, const FreeDia
         if (used[t]) o << "      " << C << " p" << t << "0 = {1, 0}, p" << t << "1 = {1, 0};\n";
     for (int f = 0; f < nfd; ++f) {
         const int mo = fd[f].mo, a = fd[f].a, b = fd[f].b;
-        const int ra = reg(a), rb = b >= 0 ? reg(b) : -1;
-        if (b < 0) {  // 1q, its bit outside the registers
+        const int ra = reg(a), rb = reg(b);
+        if (b == -1) {  // 1q, its bit outside the registers
             o << "      ff = cmul(ff, " << bitv(a) << " ? " << mat(mo + 1) << " : " << mat(mo) << ");\n";
         } else if (ra < 0 && rb < 0) {
-            o << "      { const uint32_t c = " << bitv(a) << " | (" << bitv(b) << " << 1);\n        ff = cmul(ff, (c & 2u) ? "
-              << "((c & 1u) ? " << mat(mo + 3) << " : " << mat(mo + 2) << ") : ((c & 1u) ? " << mat(mo + 1) << " : "
+            o << "      { const uint32_t cc = " << bitv(a) << " | (" << bitv(b) << " << 1);\n        ff = cmul(ff, (cc & 2u) ? "
+              << "((cc & 1u) ? " << mat(mo + 3) << " : " << mat(mo + 2) << ") : ((cc & 1u) ? " << mat(mo + 1) << " : "
               << mat(mo) << ")); }\n";
         } else if (ra >= 0) {  // matrix index m_a + 2 g_b
             o << "      { const bool g = " << bitv(b) << ";\n        p" << ra << "0 = cmul(p" << ra << "0, g ? "

@@ -45,15 +45,23 @@ uint64_t fill_low(uint64_t set, int width, int target)
 }
 
 // Ops of `masks` (qubit masks in program order) that one pass over chunk bits
-// M runs: an op runs when its qubits lie in M and no earlier op left behind
-// shares one of them.  Clears run entries of `live`; returns how many ran.
-int run_ops(const std::vector<uint64_t> &masks, std::vector<char> &live, uint64_t M)
+// M runs: an op runs when its qubits lie in M (a diagonal one, isd[i], needs
+// no chunk bits) and no earlier op left behind shares one of them.  Clears run
+// entries of `live`; returns how many non-diagonal ops ran.
+int run_ops(const std::vector<uint64_t> &masks, const std::vector<char> &isd, std::vector<char> &live, uint64_t M)
 {
     uint64_t blocked = 0;
     int ran = 0;
     for (size_t i = 0; i < masks.size(); ++i) {
         if (!live[i]) continue;
         const uint64_t m = masks[i];
+        if (isd[i]) {
+            if (m & blocked)
+                blocked |= m;
+            else
+                live[i] = 0;
+            continue;
+        }
         if ((m & blocked) || (m & ~M)) {
             blocked |= m;
             // no later op can run once every chunk bit is blocked
@@ -69,7 +77,8 @@ int run_ops(const std::vector<uint64_t> &masks, std::vector<char> &live, uint64_
 // Passes the in-order rule needs for the live ops: each pass takes ops in
 // program order while the chunk (the `low` bits plus the ops' qubits) has at
 // most K bits.
-int greedy_passes(const std::vector<uint64_t> &masks, std::vector<char> live, uint64_t low, int K)
+int greedy_passes(const std::vector<uint64_t> &masks, const std::vector<char> &isd, std::vector<char> live,
+                  uint64_t low, int K)
 {
     int passes = 0;
     for (;;) {
@@ -79,6 +88,13 @@ int greedy_passes(const std::vector<uint64_t> &masks, std::vector<char> live, ui
             if (!live[i]) continue;
             any = true;
             const uint64_t m = masks[i];
+            if (isd[i]) {  // needs no chunk bits
+                if (m & blocked)
+                    blocked |= m;
+                else
+                    live[i] = 0;
+                continue;
+            }
             if ((m & blocked) || popc(P | m) > K) {
                 blocked |= m;
                 // a full chunk whose bits are all blocked takes no later op
@@ -112,16 +128,18 @@ inline int far_bit(int elem_bytes) { return elem_bytes == 8 ? 18 : 17; }
 // run the most ops now get the look-ahead.  Returns 0 when the search does not
 // apply (all qubits fit, or too many sets).
 uint64_t search_chunk(const PlanOp *ops, const std::vector<int> &rem, uint64_t P0, int Kp, uint64_t low, int K,
-                      uint64_t far, uint64_t out_rows)
+                      uint64_t far, uint64_t out_rows, const std::vector<uint8_t> &diag)
 {
     const int free = Kp - popc(P0);
     std::vector<uint64_t> masks;
+    std::vector<char> isd;
     masks.reserve(rem.size());
     uint64_t Q = 0;
     for (int idx : rem) {
         const uint64_t m = op_mask(ops[idx]);
         masks.push_back(m);
-        Q |= m;
+        isd.push_back(diag[idx] != 0);
+        if (!diag[idx]) Q |= m;  // diagonal gates need no chunk bits
     }
     Q &= ~P0;
     const int q = popc(Q);
@@ -147,7 +165,7 @@ uint64_t search_chunk(const PlanOp *ops, const std::vector<int> &rem, uint64_t P
         for (int i = 0; i < q; ++i)
             if ((sel >> i) & 1) M |= 1ull << bit[i];
         std::fill(live.begin(), live.end(), 1);
-        const int ran = run_ops(masks, live, M);
+        const int ran = run_ops(masks, isd, live, M);
         if (ran > 0) cands.push_back({M, ran});
         const uint64_t c = sel & (~sel + 1), r = sel + c;  // next set of the same size
         sel = (((r ^ sel) >> 2) / c) | r;
@@ -161,8 +179,8 @@ uint64_t search_chunk(const PlanOp *ops, const std::vector<int> &rem, uint64_t P
         const uint64_t M = cands[k].M;
         const int ran = cands[k].ran;
         std::fill(live.begin(), live.end(), 1);
-        run_ops(masks, live, M);
-        const int rest = greedy_passes(masks, live, low, K);
+        run_ops(masks, isd, live, M);
+        const int rest = greedy_passes(masks, isd, live, low, K);
         const int nfar = popc(M & far);
         // with an output order to write (out_rows: its row bits), prefer a
         // last pass with room for them: this pass when nothing is left, else
@@ -171,7 +189,7 @@ uint64_t search_chunk(const PlanOp *ops, const std::vector<int> &rem, uint64_t P
         if (out_rows && rest <= 1) {
             uint64_t U = 0;
             for (size_t i = 0; i < masks.size(); ++i)
-                if (rest == 0 ? !live[i] : live[i]) U |= masks[i];
+                if (!isd[i] && (rest == 0 ? !live[i] : live[i])) U |= masks[i];
             nofold = rest == 0 ? popc(P0 | U | out_rows) > Kp : popc(low | U | out_rows) > K;
         }
         if (rest < best_rest ||
@@ -548,6 +566,10 @@ bool place_free(Ctx &c, StepBuilder &sb, int idx, const int *cb, int k)
     int outside = 0;
     uint64_t lock = sb.lock, m = 0;
     for (int j = 0; j < k; ++j) {
+        if (cb[j] < 0) {  // outside the chunk (FreeDiag encoding): fixed per block
+            ++outside;
+            continue;
+        }
         m |= 1ull << cb[j];
         if (sb.reg_of(cb[j]) < 0) {
             ++outside;
@@ -678,7 +700,7 @@ bool plan_passes(int n, const PlanOp *ops, int nops, const void *mats, int elem_
     for (int i = 0; i < nops; ++i) any_probe |= ops[i].kind == OP_PROBE;
     // diagonal gates may become free diagonals (specialised one-vector plans)
     std::vector<uint8_t> diag(nops, 0);
-    if (!generic && !any_probe)
+    if (!generic && !any_probe && fuse != 1 && fuse != 3)
         for (int i = 0; i < nops; ++i) diag[i] = diagonal_op(ops[i], mats, elem_bytes);
     out.fd_off.push_back(0);
     Cd pending = {1.0, 0.0};  // scalar factored out of finished steps, not yet applied
@@ -723,7 +745,7 @@ bool plan_passes(int n, const PlanOp *ops, int nops, const void *mats, int elem_
         uint64_t far = 0;  // chunk bits this pass would gather from far-apart rows
         if (in_map && out.passes.empty())
             for (int p = far_bit(elem_bytes); p < n; ++p) far |= 1ull << in_map[p];
-        const uint64_t fixed = any_probe ? 0 : search_chunk(ops, rem, P, Kp, low, K, far, out_folded ? 0 : out_rows);
+        const uint64_t fixed = any_probe ? 0 : search_chunk(ops, rem, P, Kp, low, K, far, out_folded ? 0 : out_rows, diag);
         if (fixed) P = fixed;
         int nprobe = 0;
         std::vector<int> chosen, left;
@@ -731,6 +753,18 @@ bool plan_passes(int n, const PlanOp *ops, int nops, const void *mats, int elem_
             const PlanOp &op = ops[idx];
             const uint64_t m = op_mask(op);
             const bool probe = op.kind == OP_PROBE;
+            if (diag[idx]) {
+                // a diagonal gate needs no chunk bits: a qubit outside the
+                // chunk is fixed per block.  It keeps its place after the
+                // ops left behind on its qubits.
+                if (m & blocked) {
+                    blocked |= m;
+                    left.push_back(idx);
+                } else {
+                    chosen.push_back(idx);
+                }
+                continue;
+            }
             if ((m & blocked) || popc(P | m) > Kp || (fixed && (m & ~fixed)) || (probe && nprobe >= max_probes)) {
                 blocked |= m;
                 left.push_back(idx);
@@ -740,6 +774,10 @@ bool plan_passes(int n, const PlanOp *ops, int nops, const void *mats, int elem_
             nprobe += probe;
             chosen.push_back(idx);
         }
+        if (chosen.empty() && !rem.empty()) {
+            err = "tile plan: internal: a pass with no op";
+            return false;
+        }
         if (out_map && left.empty() && !out_folded && popc(P | out_rows) <= Kp) {
             // the last pass: it also writes the output order, in rows
             P |= out_rows;
@@ -747,8 +785,11 @@ bool plan_passes(int n, const PlanOp *ops, int nops, const void *mats, int elem_
         }
         P = fill_low(P, n, Kp);
         PassDev pd;
-        int loc[64];
+        int loc[64], comp_index[64];
         make_pass(n, Kp, P, pd, loc);
+        uint64_t pd_mask = 0;
+        for (int k = 0; k < pd.K; ++k) pd_mask |= 1ull << pd.pos[k];
+        for (int j = 0; j < pd.ncomp; ++j) comp_index[pd.comp[j]] = j;
         pd.step_off = (int32_t)out.steps.size();
         pd.probe_off = (int32_t)out.probe_ids.size();
         pd.nprobe = 0;
@@ -765,15 +806,32 @@ bool plan_passes(int n, const PlanOp *ops, int nops, const void *mats, int elem_
                 int q[4], cb[4], reg[4];
                 const int k = op_qubits(op, q);
                 uint64_t m = 0;
+                bool outside = false;  // a qubit outside the chunk (diagonal gates only)
                 for (int j = 0; j < k; ++j) {
-                    cb[j] = loc[q[j]];
-                    m |= 1ull << cb[j];
+                    if ((pd_mask >> q[j]) & 1) {
+                        cb[j] = loc[q[j]];
+                        m |= 1ull << cb[j];
+                    } else {
+                        cb[j] = -2 - comp_index[q[j]];
+                        outside = true;
+                    }
                 }
                 // a non-diagonal gate waits for the free diagonals on its bits
                 const bool dg = diag[idx] != 0;
-                if ((m & sblk) || (!dg && (m & sb.endbits))) {
+                if ((m & sblk) || (!dg && (m & sb.endbits)) || (outside && sb.dense)) {
                     sblk |= m;
                     sleft.push_back(idx);
+                    continue;
+                }
+                if (outside) {
+                    const size_t mats_before = out.mats.size();
+                    if (place_free(ctx, sb, idx, cb, k)) {
+                        if (work) *work += op_work(ctx, idx);
+                    } else {
+                        out.mats.resize(mats_before);
+                        sblk |= m;
+                        sleft.push_back(idx);
+                    }
                     continue;
                 }
                 const StepCore save = sb;
@@ -877,6 +935,10 @@ bool plan_passes(int n, const PlanOp *ops, int nops, const void *mats, int elem_
                 sb.sd.scale = push_mats(out, &pending, 1);
                 pending = {1.0, 0.0};
             }
+            if (sleft.size() == srem.size()) {
+                err = "tile plan: internal: no op fits a register step";
+                return false;
+            }
             std::sort(sb.placed.begin(), sb.placed.end());
             for (auto &pr : sb.placed) out.order.push_back(pr.second);
             out.steps.push_back(sb.sd);
@@ -915,7 +977,7 @@ bool plan_passes(int n, const PlanOp *ops, int nops, const void *mats, int elem_
             if (sd.scale >= 0) cover(sd.scale, 1);
             const FreeDiag *fd = nullptr;
             const int nfd = out.step_fdiag(si, &fd);
-            for (int f = 0; f < nfd; ++f) cover(fd[f].mo, fd[f].b >= 0 ? 4 : 2);
+            for (int f = 0; f < nfd; ++f) cover(fd[f].mo, fd[f].b != -1 ? 4 : 2);
             for (int k = 0; k < kSlots; ++k)
                 if (sd.kind[k] != S1_SKIP) cover(sd.mat[k], k % kSlotsPerRound < 4 ? 8 : 4);
         }

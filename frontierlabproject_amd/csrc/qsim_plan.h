@@ -72,8 +72,11 @@ inline uint64_t op_mask(const PlanOp &op)
 // bits (specialised kernels only).  Its bits outside the registers stay
 // outside them for the rest of the step, and no later non-diagonal gate of
 // the step touches its bits, so applying it last keeps the gate order.
+// A qubit outside the pass's chunk (diagonal gates need no chunk bits) is
+// fixed per block: its value is bit m of the block's chunk index (the m-th
+// PassDev::comp position), encoded as -2 - m.
 struct FreeDiag {
-    int32_t a, b;  // chunk-local bits of the op's qubits in matrix-index order (b = -1: 1q)
+    int32_t a, b;  // the op's qubits in matrix-index order: chunk-local bit, or -2 - m (b = -1: 1q)
     int32_t mo;    // arena offset of its diagonal (2 or 4 complex entries)
 };
 
