@@ -46,12 +46,15 @@ constexpr int kPairS[6] = {1, 3, 2, 3, 3, 2};
 
 // The free diagonals of a step, after its slot rounds, with the step's
 // factor: each thread's group bits outside the registers (gcb, the chunk
-// index of its register index 0) select every factor; a diagonal on a
-// register bit t and an outside bit gives the pair (pt0, pt1) applied by
-// that register bit's value.  The amplitudes then take the products from a
-// table over the register bits used.
-void emit_free_diagonals(std::ostringstream &o, const StepDev &sd, const FreeDiag *fd, int nfd, const char *T,
-                         int nv)
+// index of its register index 0) and the block's chunk index c (bits outside
+// the chunk) select every factor; a diagonal on a register bit t and an
+// outside bit gives the pair (pt0, pt1) applied by that register bit's value.
+// Entries exactly 1 are skipped: a controlled phase diag(1, 1, 1, d) is one
+// conditional multiply (a scalar branch when its bits lie outside the chunk).
+// The amplitudes then take the products from a table over the register bits
+// used, skipping entries known to be 1.
+void emit_free_diagonals(std::ostringstream &o, const Plan &plan, const StepDev &sd, const FreeDiag *fd, int nfd,
+                         const char *T, int nv)
 {
     auto reg = [&](int bit) {
         for (int t = 0; t < 4; ++t)
@@ -65,57 +68,92 @@ void emit_free_diagonals(std::ostringstream &o, const StepDev &sd, const FreeDia
                         : "((uint32_t)(c >> " + std::to_string(-2 - bit) + ") & 1u)";
     };
     auto mat = [](int off) { return "mats[" + std::to_string(off) + "]"; };
+    auto one = [&](int off) { return plan.mats[2 * off] == 1.0 && plan.mats[2 * off + 1] == 0.0; };
     const std::string C = std::string("cx<") + T + ">";
+    // factor x <- x * (cond ? e1 : e0), the entries at arena offsets o0, o1
+    auto factor = [&](const std::string &x, const std::string &cond, int o0, int o1) {
+        if (one(o0) && one(o1)) return;
+        if (one(o0))
+            o << "      if (" << cond << ") " << x << " = cmul(" << x << ", " << mat(o1) << ");\n";
+        else
+            o << "      " << x << " = cmul(" << x << ", " << cond << " ? " << mat(o1) << " : " << mat(o0) << ");\n";
+    };
+    bool ffu = sd.scale >= 0, u0[4] = {false, false, false, false}, u1[4] = {false, false, false, false};
+    for (int f = 0; f < nfd; ++f) {
+        const int mo = fd[f].mo, ra = reg(fd[f].a), rb = reg(fd[f].b);
+        if (fd[f].b == -1 || (ra < 0 && rb < 0)) {
+            const int D = fd[f].b == -1 ? 2 : 4;
+            for (int e = 0; e < D; ++e) ffu |= !one(mo + e);
+        } else if (ra >= 0) {  // matrix index m_a + 2 g_b
+            u0[ra] |= !one(mo) || !one(mo + 2);
+            u1[ra] |= !one(mo + 1) || !one(mo + 3);
+        } else {  // matrix index g_a + 2 m_b
+            u0[rb] |= !one(mo) || !one(mo + 1);
+            u1[rb] |= !one(mo + 2) || !one(mo + 3);
+        }
+    }
     o << "    { const uint32_t gcb = ins0(ins0(ins0(ins0(g32, " << sd.sbits[0] << "), " << sd.sbits[1] << "), "
       << sd.sbits[2] << "), " << sd.sbits[3] << ");\n";
-    o << "      " << C << " ff = " << (sd.scale >= 0 ? mat(sd.scale) : C + "{1, 0}") << ";\n";
-    bool used[4] = {false, false, false, false};
-    for (int f = 0; f < nfd; ++f) {
-        const int ra = reg(fd[f].a), rb = reg(fd[f].b);
-        if (ra >= 0) used[ra] = true;
-        if (rb >= 0) used[rb] = true;
+    o << "      (void)gcb;\n";
+    if (ffu) o << "      " << C << " ff = " << (sd.scale >= 0 ? mat(sd.scale) : C + "{1, 0}") << ";\n";
+    for (int t = 0; t < 4; ++t) {
+        if (u0[t]) o << "      " << C << " p" << t << "0 = {1, 0};\n";
+        if (u1[t]) o << "      " << C << " p" << t << "1 = {1, 0};\n";
     }
-    for (int t = 0; t < 4; ++t)
-        if (used[t]) o << "      " << C << " p" << t << "0 = {1, 0}, p" << t << "1 = {1, 0};\n";
     for (int f = 0; f < nfd; ++f) {
         const int mo = fd[f].mo, a = fd[f].a, b = fd[f].b;
         const int ra = reg(a), rb = reg(b);
         if (b == -1) {  // 1q, its bit outside the registers
-            o << "      ff = cmul(ff, " << bitv(a) << " ? " << mat(mo + 1) << " : " << mat(mo) << ");\n";
+            factor("ff", bitv(a), mo, mo + 1);
         } else if (ra < 0 && rb < 0) {
-            o << "      { const uint32_t cc = " << bitv(a) << " | (" << bitv(b) << " << 1);\n        ff = cmul(ff, (cc & 2u) ? "
-              << "((cc & 1u) ? " << mat(mo + 3) << " : " << mat(mo + 2) << ") : ((cc & 1u) ? " << mat(mo + 1) << " : "
-              << mat(mo) << ")); }\n";
+            if (one(mo) && one(mo + 1) && one(mo + 2)) {
+                if (!one(mo + 3))
+                    o << "      if (" << bitv(a) << " & " << bitv(b) << ") ff = cmul(ff, " << mat(mo + 3) << ");\n";
+            } else {
+                o << "      { const uint32_t cc = " << bitv(a) << " | (" << bitv(b) << " << 1);\n        ff = cmul(ff, (cc & 2u) ? "
+                  << "((cc & 1u) ? " << mat(mo + 3) << " : " << mat(mo + 2) << ") : ((cc & 1u) ? " << mat(mo + 1)
+                  << " : " << mat(mo) << ")); }\n";
+            }
         } else if (ra >= 0) {  // matrix index m_a + 2 g_b
-            o << "      { const bool g = " << bitv(b) << ";\n        p" << ra << "0 = cmul(p" << ra << "0, g ? "
-              << mat(mo + 2) << " : " << mat(mo) << "); p" << ra << "1 = cmul(p" << ra << "1, g ? " << mat(mo + 3)
-              << " : " << mat(mo + 1) << "); }\n";
+            factor("p" + std::to_string(ra) + "0", bitv(b), mo, mo + 2);
+            factor("p" + std::to_string(ra) + "1", bitv(b), mo + 1, mo + 3);
         } else {  // matrix index g_a + 2 m_b
-            o << "      { const bool g = " << bitv(a) << ";\n        p" << rb << "0 = cmul(p" << rb << "0, g ? "
-              << mat(mo + 1) << " : " << mat(mo) << "); p" << rb << "1 = cmul(p" << rb << "1, g ? " << mat(mo + 3)
-              << " : " << mat(mo + 2) << "); }\n";
+            factor("p" + std::to_string(rb) + "0", bitv(a), mo, mo + 1);
+            factor("p" + std::to_string(rb) + "1", bitv(a), mo + 2, mo + 3);
         }
     }
     int U[4], nu = 0;
     for (int t = 0; t < 4; ++t)
-        if (used[t]) U[nu++] = t;
-    o << "      " << C << " tb[" << (1 << nu) << "];\n      tb[0] = ff;\n";
-    for (int i = 0; i < nu; ++i)
-        for (int e = 0; e < (1 << i); ++e)
-            o << "      tb[" << e + (1 << i) << "] = cmul(tb[" << e << "], p" << U[i] << "1); tb[" << e << "] = cmul(tb["
-              << e << "], p" << U[i] << "0);\n";
+        if (u0[t] || u1[t]) U[nu++] = t;
+    // tb[e]: the factor of the amplitudes whose used register bits are e;
+    // known[e]: identically 1
+    std::vector<char> known(1 << nu, 0);
+    known[0] = !ffu;
+    o << "      " << C << " tb[" << (1 << nu) << "];\n      tb[0] = " << (ffu ? "ff" : C + "{1, 0}") << ";\n";
+    for (int i = 0; i < nu; ++i) {
+        const int t = U[i];
+        for (int e = 0; e < (1 << i); ++e) {
+            const int h = e + (1 << i);
+            known[h] = known[e] && !u1[t];
+            o << "      tb[" << h << "] = " << (u1[t] ? "cmul(tb[" + std::to_string(e) + "], p" + std::to_string(t) + "1)"
+                                                      : "tb[" + std::to_string(e) + "]") << ";\n";
+            if (u0[t]) o << "      tb[" << e << "] = cmul(tb[" << e << "], p" << t << "0);\n";
+            known[e] = known[e] && !u0[t];
+        }
+    }
     const char *vec[2] = {"a", "l"};
     for (int v = 0; v < nv; ++v)
         for (int m = 0; m < 16; ++m) {
             int e = 0;
             for (int i = 0; i < nu; ++i) e |= ((m >> U[i]) & 1) << i;
+            if (known[e]) continue;
             o << "      " << vec[v] << "[" << m << "] = cmul(" << vec[v] << "[" << m << "], tb[" << e << "]);\n";
         }
     o << "    }\n";
 }
 
 void emit_step(std::ostringstream &o, const StepDev &sd, int E, const char *T, int nv, int tpb,
-               const FreeDiag *fd = nullptr, int nfd = 0)
+               const Plan *plan = nullptr, const FreeDiag *fd = nullptr, int nfd = 0)
 {
     const int NG = E >> kG;
     const bool dbl = std::string(T) == "double";
@@ -226,7 +264,7 @@ void emit_step(std::ostringstream &o, const StepDev &sd, int E, const char *T, i
             }
         }
         if (nfd > 0) {
-            emit_free_diagonals(o, sd, fd, nfd, T, nv);
+            emit_free_diagonals(o, *plan, sd, fd, nfd, T, nv);
         } else if (sd.scale >= 0) {
             o << "    { const cx<" << T << "> f = mats[" << sd.scale << "];\n";
             for (int v = 0; v < nv; ++v)
@@ -247,7 +285,7 @@ void emit_plan_step(std::ostringstream &o, const Plan &plan, int s, int E, const
 {
     const FreeDiag *fd = nullptr;
     const int nfd = plan.step_fdiag(s, &fd);
-    emit_step(o, plan.steps[s], E, T, nv, tpb, fd, nfd);
+    emit_step(o, plan.steps[s], E, T, nv, tpb, &plan, fd, nfd);
 }
 
 std::string kernel_name(const std::string &body) { return "qsim_jit_" + std::to_string(fnv1a(body)); }
