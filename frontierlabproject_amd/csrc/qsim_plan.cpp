@@ -558,9 +558,11 @@ void fold_dense(Ctx &c, StepBuilder &sb, int idx, const int *reg)
     sb.placed.push_back({(int)sb.placed.size(), idx});
 }
 
-// Place diagonal op `idx` (chunk bits cb[0..k-1]) as a free diagonal of the
-// step: at least one of its bits is not a register bit; those become locked
-// out of the registers, provided enough unlocked bits remain to fill them.
+// Place diagonal op `idx` as a free diagonal of the step.  cb[0..k-1] are its
+// qubits' chunk-local bits, or -2 - m for a qubit outside the chunk (bit m of
+// the block's chunk index).  At least one qubit must lie outside the
+// registers; its chunk bits become locked out of the registers, provided
+// enough unlocked bits remain to fill them.
 bool place_free(Ctx &c, StepBuilder &sb, int idx, const int *cb, int k)
 {
     int outside = 0;
