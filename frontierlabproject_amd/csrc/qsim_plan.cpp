@@ -700,9 +700,10 @@ bool plan_passes(int n, const PlanOp *ops, int nops, const void *mats, int elem_
     const uint64_t low = cmin ? ((1ull << cmin) - 1) : 0;
     bool any_probe = false;
     for (int i = 0; i < nops; ++i) any_probe |= ops[i].kind == OP_PROBE;
-    // diagonal gates may become free diagonals (specialised one-vector plans)
+    // diagonal gates may become free diagonals (specialised kernels); a probe
+    // is never moved past one on its qubits (it waits for the next step)
     std::vector<uint8_t> diag(nops, 0);
-    if (!generic && !any_probe && fuse != 1 && fuse != 3)
+    if (!generic && fuse != 1 && fuse != 3)
         for (int i = 0; i < nops; ++i) diag[i] = diagonal_op(ops[i], mats, elem_bytes);
     out.fd_off.push_back(0);
     Cd pending = {1.0, 0.0};  // scalar factored out of finished steps, not yet applied

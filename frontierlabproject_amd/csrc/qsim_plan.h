@@ -141,8 +141,8 @@ struct Plan {
 // generic: only the generic slot forms, no scalars taken out, and no step
 // made dense for its cost (plans for the compiled slot-program kernels, where
 // the kinds are only known at run time and code size is what limits speed).
-// Plans that are neither generic nor carry probes, in fusion modes 0, 2 and
-// 4, run diagonal 1q/2q gates as free diagonals (FreeDiag): such a gate needs
+// Plans that are not generic, in fusion modes 0, 2 and 4, run diagonal 1q/2q
+// gates as free diagonals (FreeDiag): such a gate needs
 // no chunk bits, so the chunk search and the in-order rule place only the
 // non-diagonal gates, and within a pass it costs no register step.
 // bar (fuse 2 and 4; 0: kFuseBar): the slot cost above which a 4-bit step
