@@ -1397,3 +1397,29 @@ def test_host_driver_runs_multi_qubit_unitaries(monkeypatch):
     pm, p = make_custom_pm(6)
     with pytest.raises(ValueError, match="at most 4 qubits"):
         B.run_custom_backend(pm.run(bad), p.property_set["flat_tiling_plan"], "statevector")
+
+
+def test_rocprof_trace_summary(tmp_path):
+    """The kernel-trace summariser on a synthetic rocprofv3 CSV pair: the
+    specialised chunk passes form one class whose average is total / calls,
+    and every chunk pass's duration appears in launch order."""
+    from frontierlabproject_amd.metrics import rocprof as R
+    d = tmp_path / "tr"
+    d.mkdir()
+    (d / "x_kernel_stats.csv").write_text(
+        "Name,Calls,TotalDurationNs,AverageNs,Percentage,MinNs,MaxNs,StdDev\n"
+        "qsim_jit_1,2,6000000,3000000,60.0,2000000,4000000,0\n"
+        "qsim_jit_2,1,3000000,3000000,30.0,3000000,3000000,0\n"
+        "k_other,1,1000000,1000000,10.0,1000000,1000000,0\n")
+    hdr = ("Kind,Agent_Id,Queue_Id,Stream_Id,Thread_Id,Dispatch_Id,Kernel_Id,Kernel_Name,Correlation_Id,"
+           "Start_Timestamp,End_Timestamp,LDS_Block_Size,Scratch_Size,VGPR_Count,Accum_VGPR_Count,SGPR_Count,"
+           "Workgroup_Size_X,Workgroup_Size_Y,Workgroup_Size_Z,Grid_Size_X,Grid_Size_Y,Grid_Size_Z\n")
+    rows = [("qsim_jit_1", 0, 2000000), ("qsim_jit_2", 5000000, 8000000), ("qsim_jit_1", 9000000, 13000000),
+            ("k_other", 14000000, 15000000)]
+    body = "".join(f"KERNEL_DISPATCH,1,1,1,1,{i},{i},{n},{i},{a},{b},65536,0,52,0,112,512,1,1,1024,1,1\n"
+                   for i, (n, a, b) in enumerate(rows))
+    (d / "x_kernel_trace.csv").write_text(hdr + body)
+    md = R.trace_markdown(str(d), "cmd", "title")
+    line = [ln for ln in md.splitlines() if ln.startswith("| `qsim_jit_*")][0].split("|")
+    assert int(line[2]) == 3 and int(line[3]) == 2 and abs(float(line[5]) - 3.0) < 1e-9
+    assert "2.00   3.00   4.00" in md
