@@ -200,6 +200,45 @@ def build_tile_ops(tile: Tile, qubit_map, np_dtype):
     return ops, ngates
 
 
+_SWAP = np.array([[1, 0, 0, 0], [0, 0, 1, 0], [0, 1, 0, 0], [0, 0, 0, 1]])
+
+
+def absorb_swaps(tile_ops):
+    """Run SWAP gates as relabelling instead of data movement.  A SWAP of
+    positions (a, b) exchanges which stored positions hold the tile's local
+    qubits a and b: the ops after it address the exchanged positions and the
+    op is dropped.  Returns (ops, where) with where[l] the stored position
+    holding the tile's local qubit l after the ops; a SWAP merged with another
+    gate by fusion is no longer exact and stays an op."""
+    where = None
+    out = []
+    for o in tile_ops:
+        if o[0] == "2q" and np.array_equal(np.asarray(o[3]), _SWAP):
+            if where is None:
+                where = list(range(1 + max(max(x[1:-1]) for x in tile_ops)))
+            where[o[1]], where[o[2]] = where[o[2]], where[o[1]]
+            continue
+        if where is None:
+            out.append(o)
+        else:
+            out.append((o[0], *[where[q] for q in o[1:-1]], o[-1]))
+    return out, where
+
+
+def _relabelled(actual, layout, where):
+    """The stored arrangement (position -> qubit) after a tile's absorbed
+    SWAPs, from `actual`: the tile's local qubit l is qubit layout[l], and
+    after the SWAPs it sits where the qubit at local position where[l] sat
+    (no data moves)."""
+    if where is None:
+        return list(actual)
+    pos = {q: p for p, q in enumerate(actual)}
+    out = list(actual)
+    for l, w in enumerate(where):
+        out[pos[layout[w]]] = layout[l]
+    return out
+
+
 def run_custom_backend(qc, tile_plan: List[Tile], task: str, shots: int = None,
                        use_float32: bool = False, use_cuda_graphs: bool = True,
                        tile_mode: str = "cooperative", enable_fusion: bool = True, sample_seed: int = 0,
@@ -243,32 +282,47 @@ def run_custom_backend(qc, tile_plan: List[Tile], task: str, shots: int = None,
     sv = qgpusim_cuda.StatevectorF32(num_qubits) if use_float32 else qgpusim_cuda.Statevector(num_qubits)
     identity = list(range(num_qubits))
     t0 = time.perf_counter()
-    tile_ops_all = [build_tile_ops(tile, qubit_map, np_dtype) for tile in tile_plan]
+    # per tile: the ops (fused), how many there are, and the ops that run
+    # with SWAPs absorbed as relabelling (where: see absorb_swaps)
+    prepared = []
+    for tile in tile_plan:
+        ops_t, ng = build_tile_ops(tile, qubit_map, np_dtype)
+        if ops_t and enable_fusion:
+            ops_t = fuse_tile_ops(ops_t, np_dtype)
+        run_t, where = absorb_swaps(ops_t)
+        prepared.append((run_t, ng, len(ops_t), where))
     t_apply += time.perf_counter() - t0
     # The layout changes after the last tile with ops, up to the circuit's
     # qubit order, are written by that tile's last chunk pass (one call; the
     # same result as applying the ops, then permuting).
-    last_ops = max([i for i, (ops_i, _) in enumerate(tile_ops_all) if ops_i], default=-1)
+    last_ops = max([i for i, pr in enumerate(prepared) if pr[0]], default=-1)
     mode_code = _TILE_MODE_CODES[tile_mode]
-    layout = list(identity)
+    layout = list(identity)  # the plan's layout
+    actual = list(identity)  # where the qubits are stored (absorbed SWAPs included)
     for i, tile in enumerate(tile_plan):
         if tile.layout != layout:
-            t0 = time.perf_counter()
-            if i <= last_ops:
-                sv.permute(layout, tile.layout)
-            t_perm += time.perf_counter() - t0
             num_reorders += 1
             layout = tile.layout
+        tile_ops, ng, nfused, where = prepared[i]
+        if tile_ops and actual != layout:
+            t0 = time.perf_counter()
+            sv.permute(actual, layout)
+            t_perm += time.perf_counter() - t0
+            actual = list(layout)
         t0 = time.perf_counter()
-        tile_ops, ng = tile_ops_all[i]
         num_gates += ng
+        num_fused_ops += nfused
+        after = actual if i > last_ops >= 0 else _relabelled(actual, layout, where)
         if tile_ops:
-            if enable_fusion:
-                tile_ops = fuse_tile_ops(tile_ops, np_dtype)
             tile_ops, phase, scale = fold_global_phase(tile_ops, phase, np_dtype, scale), 0.0, 1.0
-            num_fused_ops += len(tile_ops)
-            if i == last_ops and layout != identity:
-                sv.apply_tile_permute(mode_code, tile_ops, layout, identity)
+            if i == last_ops:
+                # the SWAPs of later tiles only relabel: the last pass writes
+                # the circuit's order from the arrangement after all of them
+                for j in range(i + 1, len(tile_plan)):
+                    after = _relabelled(after, tile_plan[j].layout, prepared[j][3])
+            if i == last_ops and after != identity:
+                sv.apply_tile_permute(mode_code, tile_ops, after, identity)
+                after = list(identity)
             elif tile_mode == "cooperative":
                 sv.apply_tile_cooperative(tile_ops)
             elif tile_mode == "graphed":
@@ -289,14 +343,14 @@ def run_custom_backend(qc, tile_plan: List[Tile], task: str, shots: int = None,
                 if abs(n2 - 1.0) > renormalize_tol:
                     scale = 1.0 / np.sqrt(n2)
                     renorms += 1
+        actual = after
         t_apply += time.perf_counter() - t0
 
-    if layout != identity:  # no final all-qubit tile: back to the circuit's qubit order
+    if actual != identity:  # back to the circuit's qubit order
         t0 = time.perf_counter()
-        if last_ops < 0:
-            sv.permute(layout, identity)
+        sv.permute(actual, identity)
         t_perm += time.perf_counter() - t0
-        layout = identity
+    layout = identity
     t_sync0 = time.perf_counter()
     sv.synchronize()
     t_sync = time.perf_counter() - t_sync0
