@@ -702,9 +702,15 @@ bool plan_passes(int n, const PlanOp *ops, int nops, const void *mats, int elem_
     for (int i = 0; i < nops; ++i) any_probe |= ops[i].kind == OP_PROBE;
     // diagonal gates may become free diagonals (specialised kernels); a probe
     // is never moved past one on its qubits (it waits for the next step)
-    std::vector<uint8_t> diag(nops, 0);
+    std::vector<uint8_t> diag(nops, 0), zprobe(nops, 0);
     if (!generic && fuse != 1 && fuse != 3)
-        for (int i = 0; i < nops; ++i) diag[i] = diagonal_op(ops[i], mats, elem_bytes);
+        for (int i = 0; i < nops; ++i) {
+            diag[i] = diagonal_op(ops[i], mats, elem_bytes);
+            // a probe of a diagonal generator commutes with free diagonals
+            if (ops[i].kind == OP_PROBE)
+                zprobe[i] = is0(mat_at(mats, elem_bytes, (int64_t)ops[i].mat_off + 1)) &&
+                            is0(mat_at(mats, elem_bytes, (int64_t)ops[i].mat_off + 2));
+        }
     out.fd_off.push_back(0);
     Cd pending = {1.0, 0.0};  // scalar factored out of finished steps, not yet applied
 
@@ -819,9 +825,10 @@ bool plan_passes(int n, const PlanOp *ops, int nops, const void *mats, int elem_
                         outside = true;
                     }
                 }
-                // a non-diagonal gate waits for the free diagonals on its bits
+                // a gate that does not commute with diagonals waits for the
+                // free diagonals on its bits
                 const bool dg = diag[idx] != 0;
-                if ((m & sblk) || (!dg && (m & sb.endbits)) || (outside && sb.dense)) {
+                if ((m & sblk) || (!dg && !zprobe[idx] && (m & sb.endbits)) || (outside && sb.dense)) {
                     sblk |= m;
                     sleft.push_back(idx);
                     continue;
