@@ -239,10 +239,113 @@ def _relabelled(actual, layout, where):
     return out
 
 
+def _chunk_passes(ops, num_qubits, np_dtype):
+    """Chunk passes the planner makes for an op list (host only, default
+    chunk geometry: 13 chunk bits and 4 row bits for complex64, 11 and 3 for
+    complex128)."""
+    if not ops:
+        return 0
+    from .native import qgpusim_hip as H
+    arr, arena = H.pack_ops(ops, num_qubits, np_dtype)
+    K, cmin = (13, 4) if np_dtype == np.complex64 else (11, 3)
+    return len(H.plan_describe(num_qubits, arr, len(ops), arena, K, cmin)[0])
+
+
+# Merged op lists stay bounded: at most this many ops, and on average at
+# most this many per chunk pass (each pass becomes one specialised kernel, whose
+# size grows with its ops; a state that fits one chunk would otherwise take a
+# whole circuit into one kernel).
+MERGE_MAX_OPS = 2048
+MERGE_MAX_OPS_PER_PASS = 192
+# Below this many qubits a chunk pass costs less than compiling a kernel for a
+# merged op list (which, unlike per-tile lists, rarely repeats): no merging.
+MERGE_MIN_QUBITS = 20
+
+
+def tile_program(tile_plan, qubit_map, np_dtype, num_qubits, enable_fusion=True, merge_tiles=True):
+    """The calls that run a tile plan: a list of
+    ("perm", old, new)                         layout change (no data moves),
+    ("tile", ops, tiles)                        one op list,
+    ("tile_perm", ops, old, new, changes)       the last op list, its last pass
+                                                writing the circuit's order,
+    ops in stored positions of the arrangement the call starts from, `tiles`
+    the plan tiles the call runs, and per-tile (gates, fused ops, ops run).
+
+    Consecutive tiles run as one op list when the planner needs fewer chunk
+    passes for them together than apart (a later tile's ops are addressed
+    through the first one's layout: the chunk planner, not the tile plan,
+    decides what moves where); SWAPs run as relabelling (absorb_swaps)."""
+    n = num_qubits
+    identity = list(range(n))
+    merge_tiles = merge_tiles and n >= MERGE_MIN_QUBITS
+    per_tile = []  # (layout, fused ops on logical qubits, gates, fused ops)
+    for tile in tile_plan:
+        ops, ng = build_tile_ops(tile, qubit_map, np_dtype)
+        if ops and enable_fusion:
+            ops = fuse_tile_ops(ops, np_dtype)
+        lops = [(o[0], *[tile.layout[q] for q in o[1:-1]], o[-1]) for o in ops]
+        per_tile.append((list(tile.layout), lops, ng, len(ops)))
+
+    def in_frame(lops, frame):
+        pos = {q: p for p, q in enumerate(frame)}
+        return [(o[0], *[pos[q] for q in o[1:-1]], o[-1]) for o in lops]
+
+    # groups of consecutive tiles: (frame layout, ops in frame positions, tiles)
+    groups = []
+    cur_passes = 0
+    for i, (layout, lops, _, _) in enumerate(per_tile):
+        if not lops:
+            if groups:
+                groups[-1][2].append(i)
+            continue
+        if groups and merge_tiles and len(groups[-1][1]) + len(lops) <= MERGE_MAX_OPS:
+            frame, gops, gt = groups[-1]
+            cand = gops + in_frame(lops, frame)
+            p_cand = _chunk_passes(absorb_swaps(cand)[0], n, np_dtype)
+            p_own = _chunk_passes(absorb_swaps(in_frame(lops, layout))[0], n, np_dtype)
+            if p_cand < cur_passes + p_own and len(cand) <= MERGE_MAX_OPS_PER_PASS * p_cand:
+                groups[-1] = (frame, cand, gt + [i])
+                cur_passes = p_cand
+                continue
+        groups.append((layout, in_frame(lops, layout), [i]))
+        if merge_tiles:
+            cur_passes = _chunk_passes(absorb_swaps(groups[-1][1])[0], n, np_dtype)
+    runs = [absorb_swaps(g[1]) for g in groups]
+    last = max([k for k, r in enumerate(runs) if r[0]], default=-1)
+    prog = []
+    actual = list(identity)
+    for k, (frame, _, gt) in enumerate(groups):
+        run, where = runs[k]
+        if run and actual != frame:
+            prog.append(("perm", list(actual), list(frame)))
+            actual = list(frame)
+        after = actual if k > last >= 0 else _relabelled(actual, frame, where)
+        if run:
+            if k == last:
+                # later SWAPs only relabel: the last pass writes the circuit's
+                # order from the arrangement after all of them
+                for j in range(k + 1, len(groups)):
+                    after = _relabelled(after, groups[j][0], runs[j][1])
+            if k == last and after != identity:
+                changes = sum(1 for j in range(gt[-1] + 1, len(tile_plan))
+                              if tile_plan[j].layout != tile_plan[j - 1].layout)
+                prog.append(("tile_perm", run, list(after), list(identity), max(1, changes), gt))
+                after = list(identity)
+            else:
+                prog.append(("tile", run, gt))
+        actual = after
+    if actual != identity:
+        prog.append(("perm", list(actual), list(identity)))
+    stats = [(ng, nf, sum(1 for o in absorb_swaps(in_frame(lops, layout))[0]))
+             for layout, lops, ng, nf in per_tile]
+    return prog, stats
+
+
 def run_custom_backend(qc, tile_plan: List[Tile], task: str, shots: int = None,
                        use_float32: bool = False, use_cuda_graphs: bool = True,
                        tile_mode: str = "cooperative", enable_fusion: bool = True, sample_seed: int = 0,
-                       renormalize_every: int = None, renormalize_tol: float = RENORM_TOL):
+                       renormalize_every: int = None, renormalize_tol: float = RENORM_TOL,
+                       merge_tiles: bool = True):
     """Simulate ``qc`` along ``tile_plan`` on the GPU (backend.py:336-522).
 
     Returns ``(statevector, metrics)``; ``metrics`` has the reference's keys:
@@ -260,6 +363,9 @@ def run_custom_backend(qc, tile_plan: List[Tile], task: str, shots: int = None,
     complex128): tiles with ops between FP64 norm checks; a norm off by more
     than ``renormalize_tol`` is corrected in the next tile's first matrix
     (``metrics["renorm_checks"]``, ``metrics["renormalizations"]``).
+    ``merge_tiles`` (default True): consecutive tiles run as one op list when
+    the chunk planner needs fewer passes for them together (tile_program);
+    the metrics still count per plan tile.
     """
     if tile_mode not in ("cooperative", "graphed", "sequential"):
         raise ValueError(f"unknown tile_mode {tile_mode!r}")
@@ -282,74 +388,51 @@ def run_custom_backend(qc, tile_plan: List[Tile], task: str, shots: int = None,
     sv = qgpusim_cuda.StatevectorF32(num_qubits) if use_float32 else qgpusim_cuda.Statevector(num_qubits)
     identity = list(range(num_qubits))
     t0 = time.perf_counter()
-    # per tile: the ops (fused), how many there are, and the ops that run
-    # with SWAPs absorbed as relabelling (where: see absorb_swaps)
-    prepared = []
-    for tile in tile_plan:
-        ops_t, ng = build_tile_ops(tile, qubit_map, np_dtype)
-        if ops_t and enable_fusion:
-            ops_t = fuse_tile_ops(ops_t, np_dtype)
-        run_t, where = absorb_swaps(ops_t)
-        prepared.append((run_t, ng, len(ops_t), where))
+    prog, stats = tile_program(tile_plan, qubit_map, np_dtype, num_qubits, enable_fusion, merge_tiles)
     t_apply += time.perf_counter() - t0
-    # The layout changes after the last tile with ops, up to the circuit's
-    # qubit order, are written by that tile's last chunk pass (one call; the
-    # same result as applying the ops, then permuting).
-    last_ops = max([i for i, pr in enumerate(prepared) if pr[0]], default=-1)
-    mode_code = _TILE_MODE_CODES[tile_mode]
-    layout = list(identity)  # the plan's layout
-    actual = list(identity)  # where the qubits are stored (absorbed SWAPs included)
+    # metrics per plan tile, as the reference counts them
+    layout = list(identity)
     for i, tile in enumerate(tile_plan):
         if tile.layout != layout:
             num_reorders += 1
             layout = tile.layout
-        tile_ops, ng, nfused, where = prepared[i]
-        if tile_ops and actual != layout:
-            t0 = time.perf_counter()
-            sv.permute(actual, layout)
-            t_perm += time.perf_counter() - t0
-            actual = list(layout)
-        t0 = time.perf_counter()
+        ng, nf, nrun = stats[i]
         num_gates += ng
-        num_fused_ops += nfused
-        after = actual if i > last_ops >= 0 else _relabelled(actual, layout, where)
-        if tile_ops:
-            tile_ops, phase, scale = fold_global_phase(tile_ops, phase, np_dtype, scale), 0.0, 1.0
-            if i == last_ops:
-                # the SWAPs of later tiles only relabel: the last pass writes
-                # the circuit's order from the arrangement after all of them
-                for j in range(i + 1, len(tile_plan)):
-                    after = _relabelled(after, tile_plan[j].layout, prepared[j][3])
-            if i == last_ops and after != identity:
-                sv.apply_tile_permute(mode_code, tile_ops, after, identity)
-                after = list(identity)
-            elif tile_mode == "cooperative":
-                sv.apply_tile_cooperative(tile_ops)
-            elif tile_mode == "graphed":
-                sv.apply_tile_graphed(tile_ops)
-            else:
-                sv.apply_tile_dev(tile_ops)
+        num_fused_ops += nf
+        if nrun:
             if tile_mode == "cooperative":
                 num_kernel_launches += 1
             elif tile_mode == "graphed":
                 num_graph_replays += 1
             else:
-                num_kernel_launches += len(tile_ops)
-            since_check += 1
-            if renormalize_every and since_check >= renormalize_every and i < last_ops:
-                since_check = 0
-                renorm_checks += 1
-                n2 = sv.norm_squared()
-                if abs(n2 - 1.0) > renormalize_tol:
-                    scale = 1.0 / np.sqrt(n2)
-                    renorms += 1
-        actual = after
-        t_apply += time.perf_counter() - t0
-
-    if actual != identity:  # back to the circuit's qubit order
+                num_kernel_launches += nrun
+    mode_code = _TILE_MODE_CODES[tile_mode]
+    last_call = max([k for k, e in enumerate(prog) if e[0] != "perm"], default=-1)
+    for k, entry in enumerate(prog):
         t0 = time.perf_counter()
-        sv.permute(actual, identity)
-        t_perm += time.perf_counter() - t0
+        if entry[0] == "perm":
+            sv.permute(entry[1], entry[2])
+            t_perm += time.perf_counter() - t0
+            continue
+        tile_ops = entry[1]
+        tile_ops, phase, scale = fold_global_phase(tile_ops, phase, np_dtype, scale), 0.0, 1.0
+        if entry[0] == "tile_perm":
+            sv.apply_tile_permute(mode_code, tile_ops, entry[2], entry[3])
+        elif tile_mode == "cooperative":
+            sv.apply_tile_cooperative(tile_ops)
+        elif tile_mode == "graphed":
+            sv.apply_tile_graphed(tile_ops)
+        else:
+            sv.apply_tile_dev(tile_ops)
+        since_check += sum(1 for i in entry[-1] if stats[i][2])
+        if renormalize_every and since_check >= renormalize_every and k < last_call:
+            since_check = 0
+            renorm_checks += 1
+            n2 = sv.norm_squared()
+            if abs(n2 - 1.0) > renormalize_tol:
+                scale = 1.0 / np.sqrt(n2)
+                renorms += 1
+        t_apply += time.perf_counter() - t0
     layout = identity
     t_sync0 = time.perf_counter()
     sv.synchronize()
