@@ -507,7 +507,7 @@ def fusion_sweep(hip, sv, prog, args):
     bar of modes 2 (a 4-qubit step fused when its slot arithmetic exceeds
     `bar` multiply-adds per amplitude) and 4 (a 5-qubit step kept when its
     gates' vector work exceeds `bar5`), beside mode 0; the fastest is named."""
-    cands = [(0, 0, 0)] + [(2, b, 0) for b in (16, 24, 32, 48)] + [(4, 32, b5) for b5 in (32, 48, 64, 96, 128)]
+    cands = [(0, 0, 0)] + [(2, b, 0) for b in (16, 32, 48)] + [(4, 32, b5) for b5 in (32, 64, 128)]
     rows = []
     for mode, bar, bar5 in cands:
         sv.set_fusion(mode)
