@@ -88,7 +88,10 @@ struct DescImage {
         off_step = al(off_pass + p.passes.size() * sizeof(PassDev));
         off_ids = al(off_step + p.steps.size() * sizeof(StepDev));
         off_mats = al(off_ids + p.probe_ids.size() * sizeof(int32_t));
-        bytes.assign(al(off_mats + nm * sizeof(cx<R>)) + 256, 0);
+        // the raw words follow the matrices at Plan::raw_offset from them
+        const size_t off_raw = off_mats + p.raw_offset(sizeof(cx<R>));
+        bytes.assign(al(off_raw + p.raw.size() * sizeof(uint32_t)) + 256, 0);
+        if (!p.raw.empty()) std::memcpy(&bytes[off_raw], p.raw.data(), p.raw.size() * sizeof(uint32_t));
         if (!p.passes.empty()) std::memcpy(&bytes[off_pass], p.passes.data(), p.passes.size() * sizeof(PassDev));
         if (!p.steps.empty()) std::memcpy(&bytes[off_step], p.steps.data(), p.steps.size() * sizeof(StepDev));
         if (!p.probe_ids.empty())

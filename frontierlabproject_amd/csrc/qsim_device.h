@@ -24,6 +24,15 @@ __device__ __forceinline__ cx<R> cmul(cx<R> a, cx<R> b)
     return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
 }
 
+// exp(2 pi i t) for a phase t in turns held as a 32-bit fraction (t = u / 2^32,
+// so sums of phases wrap exactly): sin and cos of pi x, x = 2t in [-1, 1).
+__device__ __forceinline__ cx<float> cis_turns(uint32_t u)
+{
+    float s, c;
+    sincospif((float)(int32_t)u * 0x1p-31f, &s, &c);
+    return {c, s};
+}
+
 template <typename R>
 __device__ __forceinline__ cx<R> cmad(cx<R> a, cx<R> b, cx<R> c)  // a*b + c
 {

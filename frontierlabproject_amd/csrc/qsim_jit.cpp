@@ -50,9 +50,11 @@ constexpr int kPairS[6] = {1, 3, 2, 3, 3, 2};
 // the chunk) select every factor; a diagonal on a register bit t and an
 // outside bit gives the pair (pt0, pt1) applied by that register bit's value.
 // Entries exactly 1 are skipped: a controlled phase diag(1, 1, 1, d) is one
-// conditional multiply (a scalar branch when its bits lie outside the chunk).
-// The amplitudes then take the products from a table over the register bits
-// used, skipping entries known to be 1.
+// conditional update (a scalar branch when its bits lie outside the chunk).
+// complex64 phases (FreeDiag::ro) add up as 32-bit turns, exact modulo a
+// turn, and become one complex factor per target (cis_turns); others multiply
+// as complex factors.  The amplitudes then take the products from a table
+// over the register bits used, skipping entries known to be 1.
 void emit_free_diagonals(std::ostringstream &o, const Plan &plan, const StepDev &sd, const FreeDiag *fd, int nfd,
                          const char *T, int nv)
 {
@@ -68,60 +70,99 @@ void emit_free_diagonals(std::ostringstream &o, const Plan &plan, const StepDev 
                         : "((uint32_t)(c >> " + std::to_string(-2 - bit) + ") & 1u)";
     };
     auto mat = [](int off) { return "mats[" + std::to_string(off) + "]"; };
+    auto rw = [](int off) { return "raw[" + std::to_string(off) + "]"; };
     auto one = [&](int off) { return plan.mats[2 * off] == 1.0 && plan.mats[2 * off + 1] == 0.0; };
     const std::string C = std::string("cx<") + T + ">";
-    // factor x <- x * (cond ? e1 : e0), the entries at arena offsets o0, o1
-    auto factor = [&](const std::string &x, const std::string &cond, int o0, int o1) {
+    // targets: 0 = ff, 1 + 2 t + s = p<t><s>; cplx: a complex factor, ang: turns
+    std::string name[9] = {"ff"};
+    for (int t = 0; t < 4; ++t)
+        for (int b = 0; b < 2; ++b) name[1 + 2 * t + b] = "p" + std::to_string(t) + std::to_string(b);
+    bool cplx[9] = {}, ang[9] = {};
+    cplx[0] = sd.scale >= 0;
+    // x <- x * (cond ? e1 : e0), entries at arena offsets o0, o1 (turns at r0, r1)
+    auto need = [&](int x, int o0, int o1, int ro) {
         if (one(o0) && one(o1)) return;
-        if (one(o0))
-            o << "      if (" << cond << ") " << x << " = cmul(" << x << ", " << mat(o1) << ");\n";
-        else
-            o << "      " << x << " = cmul(" << x << ", " << cond << " ? " << mat(o1) << " : " << mat(o0) << ");\n";
+        (ro >= 0 ? ang : cplx)[x] = true;
     };
-    bool ffu = sd.scale >= 0, u0[4] = {false, false, false, false}, u1[4] = {false, false, false, false};
+    struct Up {
+        int x, o0, o1, r0, r1;
+        std::string cond;
+    };
+    std::vector<Up> ups;
+    std::vector<std::string> quads;  // general 4-way factors of ff
     for (int f = 0; f < nfd; ++f) {
-        const int mo = fd[f].mo, ra = reg(fd[f].a), rb = reg(fd[f].b);
-        if (fd[f].b == -1 || (ra < 0 && rb < 0)) {
-            const int D = fd[f].b == -1 ? 2 : 4;
-            for (int e = 0; e < D; ++e) ffu |= !one(mo + e);
+        const int mo = fd[f].mo, a = fd[f].a, b = fd[f].b, ro = fd[f].ro;
+        const int ra = reg(a), rb = reg(b);
+        auto R = [&](int i) { return ro >= 0 ? ro + i : -1; };
+        if (b == -1) {  // 1q, its bit outside the registers
+            need(0, mo, mo + 1, ro);
+            ups.push_back({0, mo, mo + 1, R(0), R(1), bitv(a)});
+        } else if (ra < 0 && rb < 0) {
+            if (one(mo) && one(mo + 1) && one(mo + 2)) {
+                need(0, mo, mo + 3, ro);
+                ups.push_back({0, mo, mo + 3, R(0), R(3), bitv(a) + " & " + bitv(b)});
+            } else {
+                for (int e = 0; e < 4; ++e) (ro >= 0 ? ang : cplx)[0] |= !one(mo + e);
+                std::ostringstream q;
+                q << "      { const uint32_t cc = " << bitv(a) << " | (" << bitv(b) << " << 1);\n";
+                if (ro >= 0)
+                    q << "        aff += (cc & 2u) ? ((cc & 1u) ? " << rw(ro + 3) << " : " << rw(ro + 2) << ") : ((cc & 1u) ? "
+                      << rw(ro + 1) << " : " << rw(ro) << "); }\n";
+                else
+                    q << "        ff = cmul(ff, (cc & 2u) ? ((cc & 1u) ? " << mat(mo + 3) << " : " << mat(mo + 2)
+                      << ") : ((cc & 1u) ? " << mat(mo + 1) << " : " << mat(mo) << ")); }\n";
+                quads.push_back(q.str());
+            }
         } else if (ra >= 0) {  // matrix index m_a + 2 g_b
-            u0[ra] |= !one(mo) || !one(mo + 2);
-            u1[ra] |= !one(mo + 1) || !one(mo + 3);
+            need(1 + 2 * ra, mo, mo + 2, ro);
+            need(2 + 2 * ra, mo + 1, mo + 3, ro);
+            ups.push_back({1 + 2 * ra, mo, mo + 2, R(0), R(2), bitv(b)});
+            ups.push_back({2 + 2 * ra, mo + 1, mo + 3, R(1), R(3), bitv(b)});
         } else {  // matrix index g_a + 2 m_b
-            u0[rb] |= !one(mo) || !one(mo + 1);
-            u1[rb] |= !one(mo + 2) || !one(mo + 3);
+            need(1 + 2 * rb, mo, mo + 1, ro);
+            need(2 + 2 * rb, mo + 2, mo + 3, ro);
+            ups.push_back({1 + 2 * rb, mo, mo + 1, R(0), R(1), bitv(a)});
+            ups.push_back({2 + 2 * rb, mo + 2, mo + 3, R(2), R(3), bitv(a)});
         }
     }
     o << "    { const uint32_t gcb = ins0(ins0(ins0(ins0(g32, " << sd.sbits[0] << "), " << sd.sbits[1] << "), "
       << sd.sbits[2] << "), " << sd.sbits[3] << ");\n";
     o << "      (void)gcb;\n";
-    if (ffu) o << "      " << C << " ff = " << (sd.scale >= 0 ? mat(sd.scale) : C + "{1, 0}") << ";\n";
-    for (int t = 0; t < 4; ++t) {
-        if (u0[t]) o << "      " << C << " p" << t << "0 = {1, 0};\n";
-        if (u1[t]) o << "      " << C << " p" << t << "1 = {1, 0};\n";
+    bool anyang = false;
+    for (int x = 0; x < 9; ++x) anyang |= ang[x];
+    if (anyang)
+        o << "      const uint32_t *raw = reinterpret_cast<const uint32_t *>(reinterpret_cast<const char *>(mats) + "
+          << plan.raw_offset(8) << ");\n";
+    for (int x = 0; x < 9; ++x) {
+        if (cplx[x]) o << "      " << C << " " << name[x] << " = " << (x == 0 && sd.scale >= 0 ? mat(sd.scale) : C + "{1, 0}") << ";\n";
+        if (ang[x]) o << "      uint32_t a" << name[x] << " = 0u;\n";
     }
-    for (int f = 0; f < nfd; ++f) {
-        const int mo = fd[f].mo, a = fd[f].a, b = fd[f].b;
-        const int ra = reg(a), rb = reg(b);
-        if (b == -1) {  // 1q, its bit outside the registers
-            factor("ff", bitv(a), mo, mo + 1);
-        } else if (ra < 0 && rb < 0) {
-            if (one(mo) && one(mo + 1) && one(mo + 2)) {
-                if (!one(mo + 3))
-                    o << "      if (" << bitv(a) << " & " << bitv(b) << ") ff = cmul(ff, " << mat(mo + 3) << ");\n";
-            } else {
-                o << "      { const uint32_t cc = " << bitv(a) << " | (" << bitv(b) << " << 1);\n        ff = cmul(ff, (cc & 2u) ? "
-                  << "((cc & 1u) ? " << mat(mo + 3) << " : " << mat(mo + 2) << ") : ((cc & 1u) ? " << mat(mo + 1)
-                  << " : " << mat(mo) << ")); }\n";
-            }
-        } else if (ra >= 0) {  // matrix index m_a + 2 g_b
-            factor("p" + std::to_string(ra) + "0", bitv(b), mo, mo + 2);
-            factor("p" + std::to_string(ra) + "1", bitv(b), mo + 1, mo + 3);
-        } else {  // matrix index g_a + 2 m_b
-            factor("p" + std::to_string(rb) + "0", bitv(a), mo, mo + 1);
-            factor("p" + std::to_string(rb) + "1", bitv(a), mo + 2, mo + 3);
+    for (const Up &u : ups) {
+        if (one(u.o0) && one(u.o1)) continue;
+        const std::string &x = name[u.x];
+        if (u.r0 >= 0) {
+            if (one(u.o0))
+                o << "      if (" << u.cond << ") a" << x << " += " << rw(u.r1) << ";\n";
+            else
+                o << "      a" << x << " += (" << u.cond << ") ? " << rw(u.r1) << " : " << rw(u.r0) << ";\n";
+        } else if (one(u.o0)) {
+            o << "      if (" << u.cond << ") " << x << " = cmul(" << x << ", " << mat(u.o1) << ");\n";
+        } else {
+            o << "      " << x << " = cmul(" << x << ", (" << u.cond << ") ? " << mat(u.o1) << " : " << mat(u.o0) << ");\n";
         }
     }
+    for (const std::string &q : quads) o << q;
+    // one complex value per target
+    for (int x = 0; x < 9; ++x) {
+        if (!ang[x]) continue;
+        if (cplx[x])
+            o << "      " << name[x] << " = cmul(" << name[x] << ", cis_turns(a" << name[x] << "));\n";
+        else
+            o << "      const " << C << " " << name[x] << " = cis_turns(a" << name[x] << ");\n";
+    }
+    const bool ffu = cplx[0] || ang[0];
+    bool u0[4], u1[4];
+    for (int t = 0; t < 4; ++t) u0[t] = cplx[1 + 2 * t] || ang[1 + 2 * t], u1[t] = cplx[2 + 2 * t] || ang[2 + 2 * t];
     int U[4], nu = 0;
     for (int t = 0; t < 4; ++t)
         if (u0[t] || u1[t]) U[nu++] = t;

@@ -588,7 +588,18 @@ bool place_free(Ctx &c, StepBuilder &sb, int idx, const int *cb, int k)
     for (int i = 0; i < D; ++i) d[i] = mat_at(c.mats, c.eb, (int64_t)op.mat_off + (D + 1) * i);
     sb.lock = lock;
     sb.endbits |= m;
-    sb.fd.push_back({cb[0], k > 1 ? cb[1] : -1, push_mats(*c.plan, d, D)});
+    // complex64 phases (every entry of modulus 1) also as turns
+    int32_t ro = -1;
+    bool unit = c.eb == 8;
+    for (int i = 0; i < D && unit; ++i) unit = std::fabs(d[i].re * d[i].re + d[i].im * d[i].im - 1.0) < 1e-12;
+    if (unit) {
+        ro = (int32_t)c.plan->raw.size();
+        for (int i = 0; i < D; ++i) {
+            const double t = std::atan2(d[i].im, d[i].re) / (2.0 * M_PI);  // (-1/2, 1/2]
+            c.plan->raw.push_back((uint32_t)(int64_t)std::llround(t * 4294967296.0));
+        }
+    }
+    sb.fd.push_back({cb[0], k > 1 ? cb[1] : -1, push_mats(*c.plan, d, D), ro});
     sb.placed.push_back({1000 + (int)sb.fd.size(), idx});  // after every slot
     sb.cost += 1.0;
     return true;

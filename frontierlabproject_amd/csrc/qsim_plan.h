@@ -78,6 +78,7 @@ inline uint64_t op_mask(const PlanOp &op)
 struct FreeDiag {
     int32_t a, b;  // the op's qubits in matrix-index order: chunk-local bit, or -2 - m (b = -1: 1q)
     int32_t mo;    // arena offset of its diagonal (2 or 4 complex entries)
+    int32_t ro;    // complex64 unit-modulus diagonals: offset of its phases in Plan::raw, else -1
 };
 
 struct Plan {
@@ -107,6 +108,11 @@ struct Plan {
     // empty when the plan has none.
     std::vector<FreeDiag> fdiag;
     std::vector<int32_t> fd_off;
+    // 32-bit words uploaded after the matrices (at byte offset raw_offset()
+    // from the arena): the phases of complex64 free diagonals in turns, as
+    // fractions of 2^32, so that kernels add them exactly modulo a turn.
+    std::vector<uint32_t> raw;
+    size_t raw_offset(size_t amp_bytes) const { return (mats.size() / 2 * amp_bytes + 255) & ~size_t(255); }
     int step_fdiag(int s, const FreeDiag **fd) const
     {
         if (fd_off.empty()) return 0;
