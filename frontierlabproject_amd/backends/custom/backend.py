@@ -26,6 +26,12 @@ Differences from the reference, each deliberate:
   tile's, and the return to the circuit's order) are written by that tile's
   last chunk pass when it has room for them (``apply_tile_permute``), else
   recorded and applied by one permutation pass when the state is read;
+* SWAP gates run as relabelling of stored positions (``absorb_swaps``): no
+  data moves until the next layout change or the final order is written; the
+  metrics still count them;
+* for states of 20 qubits or more, consecutive tiles run as one op list when
+  the chunk planner needs fewer passes for them together (``tile_program``,
+  ``merge_tiles``); the metrics still count per plan tile;
 * there is no CuPy fallback simulator: the HIP library is required.
 
 ``tile_mode``:
